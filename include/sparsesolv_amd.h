@@ -1,0 +1,153 @@
+/*
+ * sparsesolv_amd.h -- C-ABI of the MI355X (gfx950) shifted-IC Krylov solvers.
+ *
+ * This is the drop-in boundary below the reference's C++ solver classes
+ * (include/SparseSolv/MatSolverBase.hpp:42-152, MatSolverICCG.hpp:36-76,
+ * MatSolverMRTR.hpp:36-88) and its pybind module (pybind/pybind.cpp:16-141).
+ * Plain pointers and sizes only; every entry point returns an ssgpu_status.
+ * No exception or process exit crosses this boundary: the reference's
+ * exit(1) on a missing diagonal (SprsPreconditioner.cpp:91-94) is returned
+ * as SSGPU_NO_DIAGONAL.
+ *
+ * Data: square CSR with int32 row offsets / column indices, both triangles
+ * stored, column indices ascending within each row (the reference's
+ * Eigen::SparseMatrix<T, RowMajor> storage, SparseMat.hpp:52).  Values are
+ * double (SSGPU_R64) or interleaved (re, im) double pairs (SSGPU_C128).
+ * Host arrays passed in are borrowed and copied; handles are owned by the
+ * caller and released with the matching destroy call.
+ */
+#ifndef SPARSESOLV_AMD_H
+#define SPARSESOLV_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SSGPU_OK = 0,
+    SSGPU_NOT_CONVERGED = 1,   /* solve finished without meeting the tolerance */
+    SSGPU_NO_DIAGONAL = 2,     /* a row of lower(A) does not end with its diagonal */
+    SSGPU_BAD_ARG = 3,
+    SSGPU_OOM = 4,
+    SSGPU_HIP_ERROR = 5,
+    SSGPU_NOT_SQUARE = 6,
+    SSGPU_UNSUPPORTED = 7
+} ssgpu_status;
+
+typedef enum { SSGPU_R64 = 0, SSGPU_C128 = 1 } ssgpu_dtype;
+
+/* Solver kinds: MatSolverICCG (MatSolverICCG.cpp:178-328 / 477-626),
+ * MatSolverMRTR with precondition type 0 (MatSolverMRTR_ICver.cpp:183-373)
+ * and type 1 (MatSolverMRTR_SGSver.cpp:42-282). */
+typedef enum { SSGPU_ICCG = 0, SSGPU_MRTR_IC = 1, SSGPU_MRTR_SGS = 2 } ssgpu_solver;
+
+/* Storage precision of the IC factor and of the triangular sweeps. */
+typedef enum { SSGPU_FACTOR_F64 = 0, SSGPU_FACTOR_F32 = 1 } ssgpu_precision;
+
+typedef struct ssgpu_context_s* ssgpu_context;
+typedef struct ssgpu_matrix_s* ssgpu_matrix;
+
+/* Solver settings.  Field meaning and defaults follow MatSolverBase
+ * (MatSolverBase.hpp:58-90, defaults MatSolverBase.cpp:32-47) and the
+ * solver classes (accel 1.05: MatSolverICCG.hpp:40; MRTR precondition type
+ * 1: MatSolverMRTR.hpp:43).  Fields after `accel_factor` are additions of
+ * this library. */
+typedef struct {
+    int is_init;                 /* zero x before solving (1) or use it as start (0) */
+    int is_diag_scale;           /* symmetric diagonal scaling before IC (ICCG / MRTR-IC) */
+    int is_save_best;            /* restore best iterate on failure */
+    int is_save_log;             /* record the normalized residual history */
+    int diverge_judge_type;      /* 0 none, 1 best*bad_div_val for bad_div_count_thres its */
+    double bad_div_val;
+    int bad_div_count_thres;
+    int conv_normalize_type;     /* 0 ||b||, 1 ||r0||, 2 conv_normalize_const */
+    double conv_normalize_const;
+    double accel_factor;         /* IC shift; < 0 selects the automatic shift search */
+    int factor_precision;        /* ssgpu_precision */
+    int mrtr_direction_fix;      /* 0: reference p = u + t*u (MatSolverMRTR_ICver.cpp:302-304);
+                                    1: standard MRTR p = u + t*p */
+} ssgpu_options;
+
+typedef struct {
+    int status;                  /* SSGPU_OK or SSGPU_NOT_CONVERGED on a finished solve */
+    int converged;
+    int iterations;              /* Krylov iterations run */
+    int log_len;                 /* entries of the residual log (0 unless is_save_log) */
+    double final_residual;       /* last normalized residual */
+    double accel_used;           /* IC shift actually used */
+    int pivot_fixes;             /* pivots with |s| < 1e-12 replaced (SprsPreconditioner.cpp:105-106) */
+    double setup_ms;             /* factorization + setup time on the device */
+    double iterate_ms;           /* Krylov loop time on the device */
+} ssgpu_result;
+
+void ssgpu_options_default(ssgpu_options* o);
+const char* ssgpu_status_string(int status);
+/* Thread-local text of the last error raised in this thread. */
+const char* ssgpu_last_error(void);
+
+/* Context: one device and one HIP stream (default priority). */
+int ssgpu_context_create(int device, ssgpu_context* out);
+int ssgpu_context_destroy(ssgpu_context ctx);
+int ssgpu_context_synchronize(ssgpu_context ctx);
+/* Bytes of device memory currently held by the context's matrices. */
+int ssgpu_context_resident_bytes(ssgpu_context ctx, int64_t* bytes);
+
+/* Upload a square CSR matrix and analyse its structure (lower/upper split,
+ * transpose map, level schedules).  Values are copied to the device. */
+int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr,
+                        const int32_t* col_idx, const void* values, ssgpu_matrix* out);
+/* Replace the values (same pattern) from host memory. */
+int ssgpu_matrix_update_values(ssgpu_matrix m, const void* values);
+int ssgpu_matrix_destroy(ssgpu_matrix m);
+int ssgpu_matrix_info(ssgpu_matrix m, int32_t* n, int64_t* nnz, int* dtype,
+                      int32_t* fwd_levels, int32_t* bwd_levels);
+
+/* Solve A x = b like MatSolver*::solve(conv_cri, max_ite, normB, A, b, x)
+ * (MatSolverBase.hpp:144-150): the factor is rebuilt on every call, as the
+ * reference does.  b and x are host arrays of n values (x in/out).
+ * normB is the right-hand-side norm the caller computed (the reference's
+ * overloads compute it from b on the host).  log (may be NULL) receives up to
+ * log_cap residual-log entries; res->log_len holds the full length. */
+int ssgpu_solve(ssgpu_matrix m, int solver, const ssgpu_options* opt, double conv_cri, int max_ite,
+                double normB, const void* b, void* x, double* log, int log_cap, ssgpu_result* res);
+
+/* Same, with b and x already resident in device memory (ordered on the
+ * context's stream). */
+int ssgpu_solve_device(ssgpu_matrix m, int solver, const ssgpu_options* opt, double conv_cri,
+                       int max_ite, double normB, const void* d_b, void* d_x, double* log,
+                       int log_cap, ssgpu_result* res);
+
+/* Kernel-level entry points (device pointers, context stream), used by the
+ * parity tests to check each piece against the reference's arithmetic. */
+int ssgpu_spmv_device(ssgpu_matrix m, const void* d_x, void* d_y);
+/* Shifted IC(0) on the device; copies L (lower(A) layout, n+1 / nnzL /
+ * nnzL) and diagD back to the host arrays given. */
+int ssgpu_ic_factor_host(ssgpu_matrix m, double accel, int32_t* L_row_ptr, int32_t* L_col_idx,
+                         void* L_values, void* diagD, int* pivot_fixes);
+/* z = (L D L^T)^{-1} r with the factor of the last ssgpu_ic_factor_host call. */
+int ssgpu_ic_apply_device(ssgpu_matrix m, const void* d_r, void* d_z);
+/* Plain sweeps on lower(A) / upper(A) (MatSolverBase.cpp:243-327). */
+int ssgpu_lower_sweep_device(ssgpu_matrix m, const void* d_r, void* d_z);
+int ssgpu_upper_sweep_device(ssgpu_matrix m, const void* d_r, void* d_z);
+/* Dot products: dot (non-conjugated for complex) and 2-norm. */
+int ssgpu_dot_device(ssgpu_context ctx, int dtype, int32_t n, const void* d_a, const void* d_b,
+                     double* out /* 1 or 2 doubles */);
+int ssgpu_norm_device(ssgpu_context ctx, int dtype, int32_t n, const void* d_a, double* out);
+
+/* Measurement hook: average device time (HIP events on the context stream)
+ * of `reps` back-to-back runs of one piece of the path.
+ *   which = 0 SpMV y = A x          (d_in = x, d_out = y)
+ *           1 IC forward sweep      (d_in = r, d_out = t)   needs a factor
+ *           2 IC backward sweep     (d_in = t, d_out = z)   needs a factor
+ *           3 IC apply (both)       (d_in = r, d_out = z)   needs a factor
+ *           4 shifted IC(0) factorization at shift `accel` (d_in, d_out unused)
+ * factor_precision selects the FP64 or FP32 factor copy for 1-3. */
+int ssgpu_time_kernel(ssgpu_matrix m, int which, int reps, double accel, int factor_precision,
+                      const void* d_in, void* d_out, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

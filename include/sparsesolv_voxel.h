@@ -77,6 +77,10 @@ int ssv_build(const ssv_params* p, int complex_values,
  * C = int curl Ni . Nj.  Each output is double[144], row-major. */
 void ssv_element_matrices(double h, double* K, double* M, double* C);
 
+/* Edge of every unknown: direction (0 x, 1 y, 2 z) and start node (i, j, k);
+ * each output is int32[n_unknowns]. */
+int ssv_unknown_edges(const ssv_params* p, int32_t* dir, int32_t* i, int32_t* j, int32_t* k);
+
 /* Global edge -> unknown map helpers for tests: local edge l of cell
  * (i,j,k) -> unknown index or -1 when the edge is a Dirichlet edge. */
 int32_t ssv_edge_unknown(const ssv_params* p, int i, int j, int k, int l);

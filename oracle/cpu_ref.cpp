@@ -199,21 +199,132 @@ inline cplx pmul_c(const cplx& a, const cplx& b) {
     return cplx(a.real() * b.real() + (-(a.imag() * b.imag())), a.real() * b.imag() + a.imag() * b.real());
 }
 
-inline double dotv(int n, const double* a, const double* b) { return dot_real(n, a, b); }
+// ---- optional: the GPU library's fixed-tree reduction order ---------------------
+// Mode 1 reproduces the device reductions exactly: a grid of
+// G = clamp(ceil(n/256), 1, 1024) blocks of 256 threads, thread g summing
+// elements g, g + 256G, ... from 0.0; per 64-lane wave a shuffle-down tree
+// (offsets 32..1); the 4 wave sums added in order; the G block partials then
+// reduced the same way by one 256-thread block.  Used to check that the GPU
+// path differs from the reference only through reduction order.
+int g_red_mode = 0;   // 0 = reference (Eigen) order, 1 = GPU tree order
+constexpr int kTB = 256;
+
+inline int tree_blocks(long long n) {
+    long long g = (n + kTB - 1) / kTB;
+    if (g < 1) g = 1;
+    if (g > 1024) g = 1024;
+    return (int)g;
+}
+// block_sum of 256 per-thread values
+inline double block_tree(const double* v) {
+    double w[4];
+    for (int wave = 0; wave < 4; ++wave) {
+        double lane[64];
+        for (int l = 0; l < 64; ++l) lane[l] = v[wave * 64 + l];
+        for (int off = 32; off > 0; off >>= 1)
+            for (int l = 0; l + off < 64; ++l) lane[l] = lane[l] + lane[l + off];
+        w[wave] = lane[0];
+    }
+    double r = w[0];
+    for (int k = 1; k < 4; ++k) r = r + w[k];
+    return r;
+}
+template <int K> void finalize_tree(int nb, const double* part, double* out) {
+    for (int k = 0; k < K; ++k) {
+        double th[kTB];
+        for (int t = 0; t < kTB; ++t) {
+            double acc = 0.0;
+            for (int b = t; b < nb; b += kTB) acc += part[(size_t)b * K + k];
+            th[t] = acc;
+        }
+        out[k] = block_tree(th);
+    }
+}
+// K interleaved accumulators, element contribution f(i, acc)
+template <int K, typename F> void tree_reduce(long long n, F f, double* out) {
+    const int G = tree_blocks(n);
+    std::vector<double> part((size_t)G * K);
+    std::vector<double> th((size_t)kTB * K);
+    for (int b = 0; b < G; ++b) {
+        for (int t = 0; t < kTB; ++t) {
+            double acc[K] = {};
+            for (long long i = (long long)b * kTB + t; i < n; i += (long long)G * kTB) f(i, acc);
+            for (int k = 0; k < K; ++k) th[(size_t)k * kTB + t] = acc[k];
+        }
+        for (int k = 0; k < K; ++k) part[(size_t)b * K + k] = block_tree(&th[(size_t)k * kTB]);
+    }
+    finalize_tree<K>(G, part.data(), out);
+}
+
+inline double dotv(int n, const double* a, const double* b) {
+    if (g_red_mode == 1) {
+        double r;
+        tree_reduce<1>(n, [&](long long i, double* acc) { acc[0] += a[i] * b[i]; }, &r);
+        return r;
+    }
+    return dot_real(n, a, b);
+}
 inline cplx dotv(int n, const cplx* a, const cplx* b) {   // (a.array()*b.array()).sum()
+    if (g_red_mode == 1) {
+        double r[2];
+        tree_reduce<2>(n, [&](long long i, double* acc) {
+            const cplx p = a[i] * b[i];
+            acc[0] += p.real();
+            acc[1] += p.imag();
+        }, r);
+        return cplx(r[0], r[1]);
+    }
     if (n == 0) return cplx(0, 0);
     return redux_cplx(n, [&](long long k) { return pmul_c(a[k], b[k]); });
 }
 inline double normv(int n, const double* a) {
+    if (g_red_mode == 1) {
+        double r;
+        tree_reduce<1>(n, [&](long long i, double* acc) { acc[0] += a[i] * a[i]; }, &r);
+        return std::sqrt(r);
+    }
     if (n == 0) return 0.0;
     return std::sqrt(redux_real(n, [&](long long k) { return a[k] * a[k]; }));
 }
 inline double normv(int n, const cplx* a) {
+    if (g_red_mode == 1) {
+        double r;
+        tree_reduce<1>(n, [&](long long i, double* acc) {
+            acc[0] += a[i].real() * a[i].real() + a[i].imag() * a[i].imag();
+        }, &r);
+        return std::sqrt(r);
+    }
     if (n == 0) return 0.0;
     const cplx s = redux_cplx(n, [&](long long k) {
         return cplx(a[k].real() * a[k].real(), a[k].imag() * a[k].imag());
     });
     return std::sqrt(s.real() + s.imag());
+}
+
+// p.(A p) as the GPU's fused SpMV computes it: one partial per SpMV row block
+// (<= 256 rows, <= 4096 entries unless one long row), then the finalize tree.
+template <typename T> T dot_spmv_order(const Csr<T>& A, const T* p, const T* ap) {
+    if (g_red_mode != 1) return dotv(A.n, p, ap);
+    constexpr int K = sizeof(T) / sizeof(double);
+    std::vector<double> part;
+    int r = 0;
+    while (r < A.n) {
+        int r1 = r + 1;
+        while (r1 < A.n && r1 - r < kTB && A.rp[r1 + 1] - A.rp[r] <= 4096) ++r1;
+        double th[K][kTB] = {};
+        for (int t = 0; t < kTB && r + t < r1; ++t) {
+            const T prod = p[r + t] * ap[r + t];
+            const double* q = reinterpret_cast<const double*>(&prod);
+            for (int k = 0; k < K; ++k) th[k][t] = 0.0 + q[k];
+        }
+        for (int k = 0; k < K; ++k) part.push_back(block_tree(th[k]));
+        r = r1;
+    }
+    double out[K];
+    finalize_tree<K>((int)(part.size() / K), part.data(), out);
+    T res;
+    std::memcpy(&res, out, sizeof(T));
+    return res;
 }
 
 inline double re_of(double x) { return x; }
@@ -430,7 +541,7 @@ bool iccg_main(State& st, double conv_cri, int max_ite, double normB, const T* d
     int bad = 0;
     for (int it = 0; it < max_ite; ++it) {
         spmv(A, P.data(), AP.data());
-        const T t2 = dotv(n, P.data(), AP.data());
+        const T t2 = dot_spmv_order(A, P.data(), AP.data());
         const T alpha = rur0 / t2;
         for (int i = 0; i < n; ++i) x[i] += alpha * P[i];
         for (int i = 0; i < n; ++i) R[i] -= alpha * AP[i];
@@ -493,8 +604,12 @@ bool mrtr_main(State& st, double conv_cri, int max_ite, double normB, const T* d
         }
         nu = zeta * w_r;
         const T t = eta * zeta_old / zeta;
-        P = Uv;
-        for (int i = 0; i < n; ++i) P[i] += t * P[i];
+        if (st.o.mrtr_direction_fix) {   // standard MRTR direction (labelled correction)
+            for (int i = 0; i < n; ++i) P[i] = Uv[i] + t * P[i];
+        } else {                          // reference form, MatSolverMRTR_ICver.cpp:302-304
+            P = Uv;
+            for (int i = 0; i < n; ++i) P[i] += t * P[i];
+        }
         zeta_old = zeta;
         for (int i = 0; i < n; ++i) X[i] += zeta * P[i];
         for (int i = 0; i < n; ++i) Y[i] *= eta;
@@ -651,6 +766,8 @@ int solve_t(int solver, const ref_options* opt, double conv_cri, int max_ite, do
 
 extern "C" {
 
+void ref_set_reduction_order(int mode) { g_red_mode = mode; }
+
 void ref_options_default(ref_options* o) {
     o->is_init = 1;
     o->is_diag_scale = 0;
@@ -663,6 +780,7 @@ void ref_options_default(ref_options* o) {
     o->conv_normalize_const = 1.0;
     o->accel_factor = 1.05;
     o->precondition_type = 1;
+    o->mrtr_direction_fix = 0;
 }
 
 long long ref_lower_nnz(int n, const int* rp, const int* ci) {

@@ -36,9 +36,15 @@ typedef struct {
     double conv_normalize_const; /* default 1 */
     double accel_factor;       /* MatSolverICCG.hpp:40 default 1.05 */
     int precondition_type;     /* MRTR only: 0 IC, 1 SGS (MatSolverMRTR.hpp:43 default 1) */
+    int mrtr_direction_fix;    /* IC-MRTR only: 0 reference p = u + t*u, 1 standard p = u + t*p */
 } ref_options;
 
 void ref_options_default(ref_options* o);
+
+/* Reduction order of dots and norms: 0 = the reference's (Eigen packet order,
+ * default), 1 = the GPU library's fixed tree.  Mode 1 exists so tests can show
+ * that the GPU path differs from the reference only through this order. */
+void ref_set_reduction_order(int mode);
 
 /* Number of stored entries of lower(A) (col <= row). */
 long long ref_lower_nnz(int n, const int* rp, const int* ci);

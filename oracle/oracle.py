@@ -25,11 +25,12 @@ class Options(ctypes.Structure):
         ("bad_div_val", ctypes.c_double), ("bad_div_count_thres", ctypes.c_int),
         ("conv_normalize_type", ctypes.c_int), ("conv_normalize_const", ctypes.c_double),
         ("accel_factor", ctypes.c_double), ("precondition_type", ctypes.c_int),
+        ("mrtr_direction_fix", ctypes.c_int),
     ]
 
     @classmethod
     def make(cls, **kw):
-        o = cls(1, 0, 0, 0, 0, 1000.0, 1000, 0, 1.0, 1.05, 1)
+        o = cls(1, 0, 0, 0, 0, 1000.0, 1000, 0, 1.0, 1.05, 1, 0)
         for k, v in kw.items():
             setattr(o, k, v)
         return o
@@ -62,10 +63,29 @@ def oracle():
         lib.ref_fr.argtypes = [I, I, V, V, V, V, V]
         lib.ref_bc.argtypes = [I, I, V, V, V, V, V]
         lib.ref_diag_scaling.argtypes = [I, I, V, V, V, V, V, V]
+        lib.ref_set_reduction_order.argtypes = [I]
         lib.ref_solve.argtypes = [I, I, ctypes.POINTER(Options), D, I, D, I, V, V, V, V, V, V, I,
                                   ctypes.POINTER(I)]
         _ORACLE = lib
     return _ORACLE
+
+
+def set_reduction_order(mode):
+    """0 = the reference's Eigen order (default); 1 = the GPU library's fixed tree."""
+    oracle().ref_set_reduction_order(int(mode))
+
+
+class reduction_order:
+    """Context manager selecting the oracle's reduction order."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        set_reduction_order(self.mode)
+
+    def __exit__(self, *a):
+        set_reduction_order(0)
 
 
 def reference_available():

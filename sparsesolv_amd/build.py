@@ -1,0 +1,63 @@
+"""Build recipes of the native parts (all in-tree, under sparsesolv_amd/lib).
+
+  libsparsesolv_amd.so  HIP kernels + solver drivers + C-ABI  (hipcc, gfx950)
+  libssvoxel.so         curl-curl voxel generator            (g++, OpenMP)
+
+Run as `python -m sparsesolv_amd.build` or through __graft_entry__.build().
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+INC = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off keeps every product and sum separately rounded (the
+# reference's x86-64 arithmetic); packed FP32 instructions are disabled.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+
+
+def build_hip():
+    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_capi.hip")]
+    out = os.path.join(LIB, "libsparsesolv_amd.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_voxel():
+    out = os.path.join(LIB, "libssvoxel.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", f"-I{INC}",
+          os.path.join(CSRC, "voxel_gen.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    os.makedirs(LIB, exist_ok=True)
+    build_voxel()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,114 @@
+// Device kernels of the gfx950 solver library (declarations of the launchers).
+//
+// All kernels are deterministic: every reduction uses a fixed grid and a
+// fixed tree, so the same inputs give the same bits on every run.  Per-row
+// arithmetic (SpMV, sweeps, IC factorization) is sequential within the row in
+// the reference's order, so those results are bit-identical to the
+// reference's CPU build.
+#pragma once
+
+#include "ssgpu_common.h"
+
+namespace ssgpu {
+
+constexpr int kBS = 256;          // threads per block of the streaming kernels
+constexpr int kSpmvCap = 4096;    // products staged in LDS per SpMV block
+constexpr int kMaxReduceBlocks = 1024;
+
+// --- SpMV --------------------------------------------------------------------
+// y = A x over the row blocks blk[0..nblk] (<= 256 rows, <= kSpmvCap entries
+// unless the block is one long row).  With dotv != nullptr, also writes the
+// per-block partial of sum(dotv[i] * y[i]) into partial (1 or 2 doubles per
+// block, non-conjugated).
+template <typename T>
+void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const int* ci, const T* v,
+                 const T* x, T* y, const T* dotv, double* partial);
+
+// --- level-scheduled sweeps ----------------------------------------------------
+enum SweepMode { FWD_DIV_LAST = 0, BWD_IC = 1, BWD_DIV_FIRST = 2 };
+// Rows [p0, p1) of a permuted CSR (prp/pci/pv/prow).
+//  FWD_DIV_LAST : x[i] = (rhs[i] - sum_{k<last} v_k x[c_k]) / v_last
+//  BWD_IC       : x[i] = aux[i] + dd[i] * (0 - sum_{k>first} v_k x[c_k])
+//  BWD_DIV_FIRST: x[i] = (rhs[i] - sum_{k>first} v_k x[c_k]) / v_first
+template <typename T, typename F>
+void launch_sweep(hipStream_t s, int mode, int p0, int p1, const int* prp, const int* pci, const F* pv,
+                  const int* prow, const T* rhs, const T* aux, const T* dd, T* x);
+
+// --- shifted IC(0) on one level -------------------------------------------------
+template <typename T>
+void launch_ic_level(hipStream_t s, int p0, int p1, const int* prp, const int* pci, T* pv, const int* prow,
+                     const int* ppos, T* diagD, double accel, int* nfix);
+
+// --- reductions -----------------------------------------------------------------
+// Grid size used for n elements (fixed function of n).
+int reduce_blocks(int64_t n);
+// out[k] = sum over blocks of partial[b*K + k], fixed order.
+void launch_finalize(hipStream_t s, int nb, int K, const double* partial, double* out);
+
+template <typename T> void launch_dot(hipStream_t s, int n, const T* a, const T* b, double* partial);
+template <typename T> void launch_norm2(hipStream_t s, int n, const T* a, double* partial);
+// three non-conjugated dots a.b, a.c, a.d (d may be null: then 2 dots)
+template <typename T>
+void launch_dot3(hipStream_t s, int n, const T* a, const T* b, const T* c, const T* d, double* partial);
+
+// --- vector kernels ---------------------------------------------------------------
+template <typename T> void launch_fill(hipStream_t s, int64_t n, T* x, T val);
+template <typename T> void launch_copy(hipStream_t s, int64_t n, const T* a, T* b);
+// ICCG: x += alpha p; r -= alpha q; partial |r|^2
+template <typename T>
+void launch_cg_update(hipStream_t s, int n, T alpha, const T* p, const T* q, T* x, T* r, double* partial);
+// ICCG: p = p*beta + z
+template <typename T> void launch_cg_pupdate(hipStream_t s, int n, T beta, const T* z, T* p);
+// r = b - q (or r = b if q null)
+template <typename T> void launch_residual(hipStream_t s, int n, const T* b, const T* q, T* r);
+// y = -1*x
+template <typename T> void launch_negate(hipStream_t s, int n, const T* x, T* y);
+// MRTR update group 1: p = u + t*u (fix=0) or p = u + t*p (fix=1); x += zeta p; y = y*eta + zeta v;
+// r -= y; partial |r|^2
+template <typename T>
+void launch_mrtr_update(hipStream_t s, int n, int fix, T t, T zeta, T eta, const T* u, const T* v, T* p, T* x,
+                        T* y, T* r, double* partial);
+// MRTR update group 2: z = z*eta + zeta w; u -= z
+template <typename T>
+void launch_mrtr_update2(hipStream_t s, int n, T zeta, T eta, const T* w, T* z, T* u);
+// SGS: tmp = rd - u
+template <typename T> void launch_sub(hipStream_t s, int n, const T* a, const T* b, T* c);
+// SGS: ard += u
+template <typename T> void launch_add_inplace(hipStream_t s, int n, const T* u, T* a);
+// SGS update: p = p*t2 + u; x += zeta p; y = y*eta + zeta ard; rd -= y; partial |rd|^2
+template <typename T>
+void launch_sgs_update(hipStream_t s, int n, T t2, T zeta, T eta, const T* u, const T* ard, T* p, T* x, T* y,
+                       T* rd, double* partial);
+
+// --- scaling and gathers ---------------------------------------------------------
+// s_i = 1/sqrt|a_ii| (1/DBL_EPSILON below 1e-12), bs_i = b_i * s_i
+template <typename T>
+void launch_diag_scale(hipStream_t s, int n, const int* diag_idx, const T* v, const T* b, T* sc, T* bs);
+// dst[k] = v[k] * (s_row * s_col) over all entries
+template <typename T>
+void launch_scale_matrix(hipStream_t s, int n, const int* rp, const int* ci, const T* v, const T* sc, T* dst);
+// dst[k] = src[map[k]]
+template <typename T, typename F>
+void launch_gather(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);
+// x = x / s  and x = x * s (s real-valued even for complex)
+// dst[k] = narrow(src[k]) (FP64 -> FP32 factor copies)
+template <typename T, typename F> void launch_narrow(hipStream_t s, int64_t n, const T* src, F* dst);
+template <typename T> void launch_div_scale(hipStream_t s, int n, const T* sc, T* x);
+template <typename T> void launch_mul_scale(hipStream_t s, int n, const T* sc, T* x);
+// count of diagonal entries failing the auto-shift test (real > 0 / Re > 0)
+template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d, int* count);
+
+// --- sync-free sweeps (ssgpu_sweep.hip) -------------------------------------------
+constexpr int kSfRows = 256;   // rows per chunk = threads per workgroup
+// fill a and b (either may be null) with the not-ready sentinel; zero two tickets
+template <typename T> void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets);
+// one sweep over all rows; kmax >= longest row (off-diagonals), <= 64
+template <typename T, typename F>
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int reverse, int* ticket, int* err,
+                     const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
+                     const T* rhs, const T* aux, const T* dd, T* x);
+// dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)
+template <typename T, typename F>
+void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);
+
+}  // namespace ssgpu

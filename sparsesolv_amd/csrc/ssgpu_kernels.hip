@@ -1,0 +1,558 @@
+// Device kernels of the gfx950 solver library.
+//
+// Compiled with -ffp-contract=off: every product and sum is rounded on its
+// own, as in the reference's x86-64 build, so per-row results match it bit
+// for bit.
+#include <cfloat>
+
+#include "ssgpu_kernels.h"
+
+namespace ssgpu {
+
+namespace {
+
+__device__ inline double re_part(double a) { return a; }
+__device__ inline double re_part(zc a) { return a.re; }
+
+template <typename T> struct Parts;
+template <> struct Parts<double> {
+    static constexpr int K = 1;
+    __device__ static void put(double* acc, double v) { acc[0] += v; }
+};
+template <> struct Parts<zc> {
+    static constexpr int K = 2;
+    __device__ static void put(double* acc, zc v) { acc[0] += v.re; acc[1] += v.im; }
+};
+
+__device__ inline double to_d(double a) { return a; }
+__device__ inline double to_d(float a) { return (double)a; }
+__device__ inline zc to_z(zc a) { return a; }
+__device__ inline zc to_z(fc a) { return zc{(double)a.re, (double)a.im}; }
+template <typename T, typename F> __device__ inline T widen(F a);
+template <> __device__ inline double widen<double, double>(double a) { return a; }
+template <> __device__ inline double widen<double, float>(float a) { return (double)a; }
+template <> __device__ inline zc widen<zc, zc>(zc a) { return a; }
+template <> __device__ inline zc widen<zc, fc>(fc a) { return zc{(double)a.re, (double)a.im}; }
+
+template <typename F, typename T> __device__ inline F narrow(T a);
+template <> __device__ inline double narrow<double, double>(double a) { return a; }
+template <> __device__ inline float narrow<float, double>(double a) { return (float)a; }
+template <> __device__ inline zc narrow<zc, zc>(zc a) { return a; }
+template <> __device__ inline fc narrow<fc, zc>(zc a) { return fc{(float)a.re, (float)a.im}; }
+
+// Block-wide sum of K doubles per thread; result valid in thread 0.
+// Fixed order: wave shuffles (down 32,16,...,1), then the 4 wave sums in order.
+template <int K> __device__ inline void block_sum(double* acc) {
+    __shared__ double red[K][kBS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double v = red[k][0];
+            for (int w = 1; w < kBS / 64; ++w) v += red[k][w];
+            acc[k] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- SpMV
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const T* __restrict__ v,
+                                              const T* __restrict__ x, T* __restrict__ y,
+                                              const T* __restrict__ dotv, double* __restrict__ partial) {
+    __shared__ T prod[kSpmvCap];
+    const int b = blockIdx.x;
+    const int r0 = blk[b], r1 = blk[b + 1];
+    const int e0 = rp[r0], e1 = rp[r1];
+    const int ne = e1 - e0;
+    const int r = r0 + (int)threadIdx.x;
+    T yv = Num<T>::zero();
+    if (ne <= kSpmvCap) {
+        for (int k = threadIdx.x; k < ne; k += kBS) prod[k] = Num<T>::mul(v[e0 + k], x[ci[e0 + k]]);
+        __syncthreads();
+        if (r < r1) {
+            const int e = rp[r + 1] - e0;
+            int k = rp[r] - e0;
+            T ta = Num<T>::zero(), tb = Num<T>::zero();
+            while (k < e) {
+                ta = Num<T>::add(ta, prod[k]);
+                ++k;
+                if (k < e) { tb = Num<T>::add(tb, prod[k]); ++k; }
+            }
+            yv = Num<T>::add(ta, tb);
+            y[r] = yv;
+        }
+    } else if (r < r1) {   // one long row per block
+        const int e = rp[r + 1];
+        int k = rp[r];
+        T ta = Num<T>::zero(), tb = Num<T>::zero();
+        while (k < e) {
+            ta = Num<T>::add(ta, Num<T>::mul(v[k], x[ci[k]]));
+            ++k;
+            if (k < e) { tb = Num<T>::add(tb, Num<T>::mul(v[k], x[ci[k]])); ++k; }
+        }
+        yv = Num<T>::add(ta, tb);
+        y[r] = yv;
+    }
+    if (dotv) {
+        constexpr int K = Parts<T>::K;
+        double acc[K] = {};
+        if (r < r1) Parts<T>::put(acc, Num<T>::mul(dotv[r], yv));
+        block_sum<K>(acc);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < K; ++k) partial[b * K + k] = acc[k];
+    }
+}
+
+// ---------------------------------------------------------------- sweeps
+template <typename T, typename F, int MODE>
+__global__ __launch_bounds__(kBS) void k_sweep(int p0, int p1, const int* __restrict__ prp,
+                                               const int* __restrict__ pci, const F* __restrict__ pv,
+                                               const int* __restrict__ prow, const T* __restrict__ rhs,
+                                               const T* __restrict__ aux, const T* __restrict__ dd,
+                                               T* __restrict__ x) {
+    const int p = p0 + blockIdx.x * kBS + threadIdx.x;
+    if (p >= p1) return;
+    const int row = prow[p];
+    const int a = prp[p], e = prp[p + 1];
+    if (MODE == FWD_DIV_LAST) {
+        T s = rhs[row];
+        for (int k = a; k < e - 1; ++k) s = Num<T>::sub(s, Num<T>::mul(widen<T, F>(pv[k]), x[pci[k]]));
+        x[row] = Num<T>::div(s, widen<T, F>(pv[e - 1]));
+    } else if (MODE == BWD_IC) {
+        T s = Num<T>::zero();
+        for (int k = a + 1; k < e; ++k) s = Num<T>::sub(s, Num<T>::mul(widen<T, F>(pv[k]), x[pci[k]]));
+        s = Num<T>::mul(s, dd[row]);
+        x[row] = Num<T>::add(s, aux[row]);
+    } else {
+        T s = rhs[row];
+        for (int k = a + 1; k < e; ++k) s = Num<T>::sub(s, Num<T>::mul(widen<T, F>(pv[k]), x[pci[k]]));
+        x[row] = Num<T>::div(s, widen<T, F>(pv[a]));
+    }
+}
+
+// ---------------------------------------------------------------- IC(0)
+__device__ inline double absval(double a) { return fabs(a); }
+__device__ inline double absval(zc a) { return hypot(a.re, a.im); }
+__device__ inline double mulacc(double v, double a) { return v * a; }
+__device__ inline zc mulacc(zc v, double a) { return v * a; }
+__device__ inline double recip(double s) { return 1.0 / s; }
+__device__ inline zc recip(zc s) { return zdiv(zc{1.0, 0.0}, s); }
+template <typename T> __device__ inline T from_real(double a);
+template <> __device__ inline double from_real<double>(double a) { return a; }
+template <> __device__ inline zc from_real<zc>(double a) { return zc{a, 0.0}; }
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_ic_level(int p0, int p1, const int* __restrict__ prp,
+                                                  const int* __restrict__ pci, T* __restrict__ pv,
+                                                  const int* __restrict__ prow, const int* __restrict__ ppos,
+                                                  T* __restrict__ diagD, double accel, int* __restrict__ nfix) {
+    const int p = p0 + blockIdx.x * kBS + threadIdx.x;
+    if (p >= p1) return;
+    const int i = prow[p];
+    const int rs = prp[p], re = prp[p + 1];
+    for (int jj = rs; jj < re; ++jj) {
+        const int j = pci[jj];
+        if (j >= i) break;
+        T s = pv[jj];
+        const int q = ppos[j];
+        int ll = prp[q];
+        const int le = prp[q + 1];
+        for (int kk = rs; kk < re; ++kk) {
+            const int ck = pci[kk];
+            if (!(ck <= j - 1)) break;
+            while (ll < le && pci[ll] < ck) ++ll;
+            if (ll < le && pci[ll] == ck)
+                s = Num<T>::sub(s, Num<T>::mul(Num<T>::mul(pv[kk], pv[ll]), diagD[ck]));
+        }
+        pv[jj] = s;
+    }
+    const int last = re - 1;
+    T s = mulacc(pv[last], accel);
+    for (int kk = rs; kk < last; ++kk) {
+        if (pci[kk] <= i - 1) s = Num<T>::sub(s, Num<T>::mul(Num<T>::mul(pv[kk], pv[kk]), diagD[pci[kk]]));
+        else break;
+    }
+    pv[last] = s;
+    if (absval(s) < 1.0e-12) {
+        diagD[i] = from_real<T>(1.0 / DBL_EPSILON);
+        atomicAdd(nfix, 1);
+    } else {
+        diagD[i] = recip(s);
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                             double* __restrict__ partial) {
+    constexpr int K = Parts<T>::K;
+    double acc[K] = {};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) Parts<T>::put(acc, Num<T>::mul(a[i], b[i]));
+    block_sum<K>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) partial[blockIdx.x * K + k] = acc[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_norm2(int n, const T* __restrict__ a, double* __restrict__ partial) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) acc[0] += Num<T>::abs2(a[i]);
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_dot3(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                              const T* __restrict__ c, const T* __restrict__ d,
+                                              double* __restrict__ partial) {
+    constexpr int K = Parts<T>::K;
+    double acc[3 * K] = {};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T ai = a[i];
+        Parts<T>::put(acc, Num<T>::mul(ai, b[i]));
+        Parts<T>::put(acc + K, Num<T>::mul(ai, c[i]));
+        if (d) Parts<T>::put(acc + 2 * K, Num<T>::mul(ai, d[i]));
+    }
+    block_sum<3 * K>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3 * K; ++k) partial[blockIdx.x * 3 * K + k] = acc[k];
+}
+
+__global__ __launch_bounds__(kBS) void k_finalize(int nb, int K, const double* __restrict__ partial,
+                                                  double* __restrict__ out) {
+    for (int k = 0; k < K; ++k) {
+        double acc[1] = {0.0};
+        for (int b = threadIdx.x; b < nb; b += kBS) acc[0] += partial[b * K + k];
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) out[k] = acc[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- vectors
+template <typename T> __global__ void k_fill(int64_t n, T* x, T val) {
+    for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) x[i] = val;
+}
+template <typename T> __global__ void k_copy(int64_t n, const T* __restrict__ a, T* __restrict__ b) {
+    for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) b[i] = a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_cg_update(int n, T alpha, const T* __restrict__ p, const T* __restrict__ q,
+                                                   T* __restrict__ x, T* __restrict__ r, double* __restrict__ partial) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        x[i] = Num<T>::add(x[i], Num<T>::mul(alpha, p[i]));
+        const T ri = Num<T>::sub(r[i], Num<T>::mul(alpha, q[i]));
+        r[i] = ri;
+        acc[0] += Num<T>::abs2(ri);
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+}
+
+template <typename T> __global__ void k_cg_pupdate(int n, T beta, const T* __restrict__ z, T* __restrict__ p) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS)
+        p[i] = Num<T>::add(Num<T>::mul(p[i], beta), z[i]);
+}
+
+template <typename T> __global__ void k_residual(int n, const T* __restrict__ b, const T* __restrict__ q, T* __restrict__ r) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) r[i] = q ? Num<T>::sub(b[i], q[i]) : b[i];
+}
+
+__device__ inline double neg(double a) { return -1.0 * a; }
+__device__ inline zc neg(zc a) { return a * -1.0; }
+template <typename T> __global__ void k_negate(int n, const T* __restrict__ x, T* __restrict__ y) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) y[i] = neg(x[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_mrtr_update(int n, int fix, T t, T zeta, T eta, const T* __restrict__ u,
+                                                     const T* __restrict__ v, T* __restrict__ p, T* __restrict__ x,
+                                                     T* __restrict__ y, T* __restrict__ r, double* __restrict__ partial) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T ui = u[i];
+        const T pi = fix ? Num<T>::add(ui, Num<T>::mul(t, p[i])) : Num<T>::add(ui, Num<T>::mul(t, ui));
+        p[i] = pi;
+        x[i] = Num<T>::add(x[i], Num<T>::mul(zeta, pi));
+        const T yi = Num<T>::add(Num<T>::mul(y[i], eta), Num<T>::mul(zeta, v[i]));
+        y[i] = yi;
+        const T ri = Num<T>::sub(r[i], yi);
+        r[i] = ri;
+        acc[0] += Num<T>::abs2(ri);
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+}
+
+template <typename T>
+__global__ void k_mrtr_update2(int n, T zeta, T eta, const T* __restrict__ w, T* __restrict__ z, T* __restrict__ u) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T zi = Num<T>::add(Num<T>::mul(z[i], eta), Num<T>::mul(zeta, w[i]));
+        z[i] = zi;
+        u[i] = Num<T>::sub(u[i], zi);
+    }
+}
+
+template <typename T> __global__ void k_sub(int n, const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ c) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) c[i] = Num<T>::sub(a[i], b[i]);
+}
+template <typename T> __global__ void k_add_inplace(int n, const T* __restrict__ u, T* __restrict__ a) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) a[i] = Num<T>::add(a[i], u[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_sgs_update(int n, T t2, T zeta, T eta, const T* __restrict__ u,
+                                                    const T* __restrict__ ard, T* __restrict__ p, T* __restrict__ x,
+                                                    T* __restrict__ y, T* __restrict__ rd, double* __restrict__ partial) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T pi = Num<T>::add(Num<T>::mul(p[i], t2), u[i]);
+        p[i] = pi;
+        x[i] = Num<T>::add(x[i], Num<T>::mul(zeta, pi));
+        const T yi = Num<T>::add(Num<T>::mul(y[i], eta), Num<T>::mul(zeta, ard[i]));
+        y[i] = yi;
+        const T ri = Num<T>::sub(rd[i], yi);
+        rd[i] = ri;
+        acc[0] += Num<T>::abs2(ri);
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+}
+
+// ---------------------------------------------------------------- scaling
+__device__ inline double diag_mag(double a) { return fabs(a); }
+__device__ inline double diag_mag(zc a) { return a.re * a.re + a.im * a.im; }   // std::norm, then sqrt
+
+template <typename T>
+__global__ void k_diag_scale(int n, const int* __restrict__ diag_idx, const T* __restrict__ v, const T* __restrict__ b,
+                             T* __restrict__ sc, T* __restrict__ bs) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const int k = diag_idx[i];
+        double t2 = 0.0;
+        if (k >= 0) {
+            const double t1b = sqrt(diag_mag(v[k]));
+            t2 = (t1b < 1.0e-12) ? 1.0 / DBL_EPSILON : 1.0 / t1b;
+        }
+        const T s = from_real<T>(t2);
+        sc[i] = s;
+        bs[i] = Num<T>::mul(b[i], s);
+    }
+}
+
+__device__ inline double sval(double a) { return a; }
+__device__ inline double sval(zc a) { return a.re; }
+__device__ inline double scl(double v, double f) { return v * f; }
+__device__ inline zc scl(zc v, double f) { return zc{v.re * f, v.im * f}; }
+
+template <typename T>
+__global__ void k_scale_matrix(int n, const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ v,
+                               const T* __restrict__ sc, T* __restrict__ dst) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const double di = sval(sc[i]);
+        for (int k = rp[i]; k < rp[i + 1]; ++k) dst[k] = scl(v[k], di * sval(sc[ci[k]]));
+    }
+}
+
+template <typename T, typename F>
+__global__ void k_gather(int64_t n, const int* __restrict__ map, const T* __restrict__ src, F* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
+        dst[i] = narrow<F, T>(src[map[i]]);
+}
+
+template <typename T, typename F>
+__global__ void k_narrow(int64_t n, const T* __restrict__ src, F* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS)
+        dst[i] = narrow<F, T>(src[i]);
+}
+
+template <typename T> __global__ void k_div_scale(int n, const T* __restrict__ sc, T* __restrict__ x) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) x[i] = Num<T>::div(x[i], sc[i]);
+}
+template <typename T> __global__ void k_mul_scale(int n, const T* __restrict__ sc, T* __restrict__ x) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) x[i] = Num<T>::mul(x[i], sc[i]);
+}
+
+__device__ inline bool nonpos(double d) { return !(d > 0); }
+__device__ inline bool nonpos(zc d) { return !(d.re > 0); }
+template <typename T> __global__ void k_count_nonpos(int n, const T* __restrict__ d, int* count) {
+    int c = 0;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) c += nonpos(d[i]) ? 1 : 0;
+    if (c) atomicAdd(count, c);
+}
+
+inline int grid_for(int64_t n, int cap = 4096) {
+    int64_t g = (n + kBS - 1) / kBS;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+int reduce_blocks(int64_t n) { return grid_for(n, kMaxReduceBlocks); }
+
+void launch_finalize(hipStream_t s, int nb, int K, const double* partial, double* out) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, s, nb, K, partial, out);
+}
+
+template <typename T>
+void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const int* ci, const T* v, const T* x, T* y,
+                 const T* dotv, double* partial) {
+    hipLaunchKernelGGL(k_spmv<T>, dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
+}
+
+template <typename T, typename F>
+void launch_sweep(hipStream_t s, int mode, int p0, int p1, const int* prp, const int* pci, const F* pv,
+                  const int* prow, const T* rhs, const T* aux, const T* dd, T* x) {
+    const int nr = p1 - p0;
+    if (nr <= 0) return;
+    const dim3 g((nr + kBS - 1) / kBS);
+    if (mode == FWD_DIV_LAST)
+        hipLaunchKernelGGL((k_sweep<T, F, FWD_DIV_LAST>), g, dim3(kBS), 0, s, p0, p1, prp, pci, pv, prow, rhs, aux, dd, x);
+    else if (mode == BWD_IC)
+        hipLaunchKernelGGL((k_sweep<T, F, BWD_IC>), g, dim3(kBS), 0, s, p0, p1, prp, pci, pv, prow, rhs, aux, dd, x);
+    else
+        hipLaunchKernelGGL((k_sweep<T, F, BWD_DIV_FIRST>), g, dim3(kBS), 0, s, p0, p1, prp, pci, pv, prow, rhs, aux, dd, x);
+}
+
+template <typename T>
+void launch_ic_level(hipStream_t s, int p0, int p1, const int* prp, const int* pci, T* pv, const int* prow,
+                     const int* ppos, T* diagD, double accel, int* nfix) {
+    const int nr = p1 - p0;
+    if (nr <= 0) return;
+    hipLaunchKernelGGL(k_ic_level<T>, dim3((nr + kBS - 1) / kBS), dim3(kBS), 0, s, p0, p1, prp, pci, pv, prow, ppos,
+                       diagD, accel, nfix);
+}
+
+template <typename T> void launch_dot(hipStream_t s, int n, const T* a, const T* b, double* partial) {
+    hipLaunchKernelGGL(k_dot<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, a, b, partial);
+}
+template <typename T> void launch_norm2(hipStream_t s, int n, const T* a, double* partial) {
+    hipLaunchKernelGGL(k_norm2<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, a, partial);
+}
+template <typename T>
+void launch_dot3(hipStream_t s, int n, const T* a, const T* b, const T* c, const T* d, double* partial) {
+    hipLaunchKernelGGL(k_dot3<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, a, b, c, d, partial);
+}
+
+template <typename T> void launch_fill(hipStream_t s, int64_t n, T* x, T val) {
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, x, val);
+}
+template <typename T> void launch_copy(hipStream_t s, int64_t n, const T* a, T* b) {
+    hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b);
+}
+template <typename T>
+void launch_cg_update(hipStream_t s, int n, T alpha, const T* p, const T* q, T* x, T* r, double* partial) {
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, alpha, p, q, x, r, partial);
+}
+template <typename T> void launch_cg_pupdate(hipStream_t s, int n, T beta, const T* z, T* p) {
+    hipLaunchKernelGGL(k_cg_pupdate<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, beta, z, p);
+}
+template <typename T> void launch_residual(hipStream_t s, int n, const T* b, const T* q, T* r) {
+    hipLaunchKernelGGL(k_residual<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, b, q, r);
+}
+template <typename T> void launch_negate(hipStream_t s, int n, const T* x, T* y) {
+    hipLaunchKernelGGL(k_negate<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, x, y);
+}
+template <typename T>
+void launch_mrtr_update(hipStream_t s, int n, int fix, T t, T zeta, T eta, const T* u, const T* v, T* p, T* x, T* y,
+                        T* r, double* partial) {
+    hipLaunchKernelGGL(k_mrtr_update<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, fix, t, zeta, eta, u, v, p, x, y,
+                       r, partial);
+}
+template <typename T> void launch_mrtr_update2(hipStream_t s, int n, T zeta, T eta, const T* w, T* z, T* u) {
+    hipLaunchKernelGGL(k_mrtr_update2<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, zeta, eta, w, z, u);
+}
+template <typename T> void launch_sub(hipStream_t s, int n, const T* a, const T* b, T* c) {
+    hipLaunchKernelGGL(k_sub<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, c);
+}
+template <typename T> void launch_add_inplace(hipStream_t s, int n, const T* u, T* a) {
+    hipLaunchKernelGGL(k_add_inplace<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, u, a);
+}
+template <typename T>
+void launch_sgs_update(hipStream_t s, int n, T t2, T zeta, T eta, const T* u, const T* ard, T* p, T* x, T* y, T* rd,
+                       double* partial) {
+    hipLaunchKernelGGL(k_sgs_update<T>, dim3(reduce_blocks(n)), dim3(kBS), 0, s, n, t2, zeta, eta, u, ard, p, x, y, rd,
+                       partial);
+}
+template <typename T>
+void launch_diag_scale(hipStream_t s, int n, const int* diag_idx, const T* v, const T* b, T* sc, T* bs) {
+    hipLaunchKernelGGL(k_diag_scale<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, diag_idx, v, b, sc, bs);
+}
+template <typename T>
+void launch_scale_matrix(hipStream_t s, int n, const int* rp, const int* ci, const T* v, const T* sc, T* dst) {
+    hipLaunchKernelGGL(k_scale_matrix<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, rp, ci, v, sc, dst);
+}
+template <typename T, typename F> void launch_gather(hipStream_t s, int64_t n, const int* map, const T* src, F* dst) {
+    hipLaunchKernelGGL((k_gather<T, F>), dim3(grid_for(n)), dim3(kBS), 0, s, n, map, src, dst);
+}
+template <typename T, typename F> void launch_narrow(hipStream_t s, int64_t n, const T* src, F* dst) {
+    hipLaunchKernelGGL((k_narrow<T, F>), dim3(grid_for(n)), dim3(kBS), 0, s, n, src, dst);
+}
+template <typename T> void launch_div_scale(hipStream_t s, int n, const T* sc, T* x) {
+    hipLaunchKernelGGL(k_div_scale<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, sc, x);
+}
+template <typename T> void launch_mul_scale(hipStream_t s, int n, const T* sc, T* x) {
+    hipLaunchKernelGGL(k_mul_scale<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, sc, x);
+}
+template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d, int* count) {
+    hipLaunchKernelGGL(k_count_nonpos<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, d, count);
+}
+
+// explicit instantiations
+#define SSGPU_INST(T)                                                                                          \
+    template void launch_spmv<T>(hipStream_t, int, const int*, const int*, const int*, const T*, const T*, T*,  \
+                                 const T*, double*);                                                           \
+    template void launch_ic_level<T>(hipStream_t, int, int, const int*, const int*, T*, const int*, const int*, \
+                                     T*, double, int*);                                                        \
+    template void launch_dot<T>(hipStream_t, int, const T*, const T*, double*);                                \
+    template void launch_norm2<T>(hipStream_t, int, const T*, double*);                                        \
+    template void launch_dot3<T>(hipStream_t, int, const T*, const T*, const T*, const T*, double*);           \
+    template void launch_fill<T>(hipStream_t, int64_t, T*, T);                                                 \
+    template void launch_copy<T>(hipStream_t, int64_t, const T*, T*);                                          \
+    template void launch_cg_update<T>(hipStream_t, int, T, const T*, const T*, T*, T*, double*);               \
+    template void launch_cg_pupdate<T>(hipStream_t, int, T, const T*, T*);                                     \
+    template void launch_residual<T>(hipStream_t, int, const T*, const T*, T*);                                \
+    template void launch_negate<T>(hipStream_t, int, const T*, T*);                                            \
+    template void launch_mrtr_update<T>(hipStream_t, int, int, T, T, T, const T*, const T*, T*, T*, T*, T*,     \
+                                        double*);                                                              \
+    template void launch_mrtr_update2<T>(hipStream_t, int, T, T, const T*, T*, T*);                            \
+    template void launch_sub<T>(hipStream_t, int, const T*, const T*, T*);                                     \
+    template void launch_add_inplace<T>(hipStream_t, int, const T*, T*);                                       \
+    template void launch_sgs_update<T>(hipStream_t, int, T, T, T, const T*, const T*, T*, T*, T*, T*, double*); \
+    template void launch_diag_scale<T>(hipStream_t, int, const int*, const T*, const T*, T*, T*);              \
+    template void launch_scale_matrix<T>(hipStream_t, int, const int*, const int*, const T*, const T*, T*);    \
+    template void launch_div_scale<T>(hipStream_t, int, const T*, T*);                                         \
+    template void launch_mul_scale<T>(hipStream_t, int, const T*, T*);                                         \
+    template void launch_count_nonpos<T>(hipStream_t, int, const T*, int*);
+
+SSGPU_INST(double)
+SSGPU_INST(zc)
+template void launch_sweep<double, double>(hipStream_t, int, int, int, const int*, const int*, const double*,
+                                           const int*, const double*, const double*, const double*, double*);
+template void launch_sweep<double, float>(hipStream_t, int, int, int, const int*, const int*, const float*,
+                                          const int*, const double*, const double*, const double*, double*);
+template void launch_sweep<zc, zc>(hipStream_t, int, int, int, const int*, const int*, const zc*, const int*,
+                                   const zc*, const zc*, const zc*, zc*);
+template void launch_sweep<zc, fc>(hipStream_t, int, int, int, const int*, const int*, const fc*, const int*,
+                                   const zc*, const zc*, const zc*, zc*);
+template void launch_narrow<double, float>(hipStream_t, int64_t, const double*, float*);
+template void launch_narrow<zc, fc>(hipStream_t, int64_t, const zc*, fc*);
+template void launch_fill<int>(hipStream_t, int64_t, int*, int);
+template void launch_gather<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
+template void launch_gather<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
+template void launch_gather<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);
+template void launch_gather<zc, fc>(hipStream_t, int64_t, const int*, const zc*, fc*);
+
+}  // namespace ssgpu

@@ -1,0 +1,274 @@
+// Sync-free level-free triangular sweeps for gfx950.
+//
+// One launch per sweep.  Rows are cut into chunks of 256 consecutive rows
+// (in the matrix's own order); a 256-thread workgroup takes a ticket from an
+// atomic counter and owns the chunk with that ticket (ascending chunks for a
+// forward sweep, descending for a backward one), one row per lane.  Because
+// tickets are handed out in dependency order, every chunk a workgroup waits
+// on belongs to a workgroup that is already running, so the sweep cannot
+// deadlock however the hardware dispatches workgroups.
+//
+// A lane keeps its row's off-diagonal entries in registers (loaded from a
+// sliced-ELL copy, 64 rows per slice, entry k of lane l at slice_base +
+// 64*k + l, so each load instruction is one coalesced 512-byte access) and
+// resolves its inputs as they appear: rows of its own chunk through LDS,
+// rows of other chunks through agent-scope atomic loads of the output
+// vector, which the prepare kernel fills with a NaN sentinel (the value is
+// its own ready flag; a computed value never has this bit pattern because
+// arithmetic only produces quiet NaNs).  Once every input is present the lane
+// forms the row's result with the reference's sequential summation order and
+// publishes it to LDS and to global memory.  A spin limit bounds every wait:
+// a lane that exceeds it sets the error flag and leaves, so the grid always
+// drains.
+#include "ssgpu_kernels.h"
+
+namespace ssgpu {
+
+namespace {
+
+constexpr unsigned long long kSentinel = 0xFFF40000DEADBEEFull;
+constexpr long kSpinLimit = 1L << 23;
+
+__device__ inline bool sent(double v) { return (unsigned long long)__double_as_longlong(v) == kSentinel; }
+__device__ inline bool sent(zc v) { return sent(v.re) || sent(v.im); }
+__device__ inline double sent_val(double*) { return __longlong_as_double((long long)kSentinel); }
+__device__ inline zc sent_val(zc*) {
+    const double s = __longlong_as_double((long long)kSentinel);
+    return zc{s, s};
+}
+
+__device__ inline double gload(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load((unsigned long long*)p, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline zc gload(const zc* p) {
+    const double* d = reinterpret_cast<const double*>(p);
+    return zc{gload(d), gload(d + 1)};
+}
+__device__ inline void gstore(double* p, double v) {
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void gstore(zc* p, zc v) {
+    double* d = reinterpret_cast<double*>(p);
+    gstore(d, v.re);
+    gstore(d + 1, v.im);
+}
+// Plain LDS reads: the spin loop starts every iteration with a compiler
+// memory barrier, so values are re-read each time, and an aligned 8-byte LDS
+// access is single-copy atomic on CDNA.
+__device__ inline double lload(const double* p) { return *p; }
+__device__ inline zc lload(const zc* p) {
+    const double* d = reinterpret_cast<const double*>(p);
+    return zc{lload(d), lload(d + 1)};
+}
+__device__ inline void lstore(double* p, double v) {
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lstore(zc* p, zc v) {
+    double* d = reinterpret_cast<double*>(p);
+    lstore(d, v.re);
+    lstore(d + 1, v.im);
+}
+
+template <typename T, typename F> __device__ inline T wid(F a);
+template <> __device__ inline double wid<double, double>(double a) { return a; }
+template <> __device__ inline double wid<double, float>(float a) { return (double)a; }
+template <> __device__ inline zc wid<zc, zc>(zc a) { return a; }
+template <> __device__ inline zc wid<zc, fc>(fc a) { return zc{(double)a.re, (double)a.im}; }
+
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int reverse, int* __restrict__ ticket,
+                                                      int* __restrict__ err, const int* __restrict__ slice_off,
+                                                      const int* __restrict__ rlen, const int* __restrict__ ecol,
+                                                      const F* __restrict__ eval, const F* __restrict__ dval,
+                                                      const T* __restrict__ rhs, const T* __restrict__ aux,
+                                                      const T* __restrict__ dd, T* __restrict__ x) {
+    __shared__ T tl[kSfRows];
+    __shared__ int s_chunk;
+    if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1);
+    tl[threadIdx.x] = sent_val((T*)nullptr);
+    __syncthreads();
+    const int t = s_chunk;
+    if (t >= nchunks) return;
+    const int c = reverse ? nchunks - 1 - t : t;
+    const int c0 = c * kSfRows;
+    const int i = c0 + (int)threadIdx.x;
+    const bool active = i < n;
+    const int lane = threadIdx.x & 63;
+    int len = 0;
+    int cols[KMAX];
+    F vals[KMAX];
+    T g[KMAX];
+    T r_i = Num<T>::zero(), a_i = Num<T>::zero(), d_i = Num<T>::zero();
+    F dg = F();
+    if (active) {
+        len = rlen[i];
+        const int base = slice_off[i >> 6] + lane;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            if (k < len) {
+                cols[k] = ecol[base + 64 * k];
+                vals[k] = eval[base + 64 * k];
+            }
+        if (MODE == BWD_IC) {
+            a_i = aux[i];
+            d_i = dd[i];
+        } else {
+            r_i = rhs[i];
+            dg = dval[i];
+        }
+    }
+    unsigned long long miss = active ? (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) : 0ull;
+    bool done = !active;
+    long spins = 0;
+    while (true) {
+        __asm__ __volatile__("" ::: "memory");
+        bool progress = false;
+        if (!done) {
+            // one generic (flat) load per missing input, pointing into LDS for
+            // rows of this chunk and into global memory otherwise: no branches,
+            // so all loads are in flight together before the single wait
+            T v[KMAX];
+            const unsigned long long want = miss;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                if (want & (1ull << k)) {
+                    const int jl = cols[k] - c0;
+                    const T* p = (jl >= 0 && jl < kSfRows) ? (const T*)&tl[jl] : x + cols[k];
+                    v[k] = gload(p);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                if ((want & (1ull << k)) && !sent(v[k])) {
+                    g[k] = v[k];
+                    miss &= ~(1ull << k);
+                }
+            }
+            if (miss == 0ull) {
+                T s = (MODE == BWD_IC) ? Num<T>::zero() : r_i;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (k < len) s = Num<T>::sub(s, Num<T>::mul(wid<T, F>(vals[k]), g[k]));
+                T out;
+                if (MODE == BWD_IC) out = Num<T>::add(Num<T>::mul(s, d_i), a_i);
+                else out = Num<T>::div(s, wid<T, F>(dg));
+                lstore(&tl[threadIdx.x], out);
+                gstore(x + i, out);
+                done = true;
+                progress = true;
+            }
+        }
+        if (__all(done)) break;
+        if (++spins > kSpinLimit) {
+            atomicOr(err, 1);
+            break;
+        }
+        if (!__any(progress)) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <typename T>
+__global__ void k_sf_prepare(int n, T* __restrict__ a, T* __restrict__ b, int* __restrict__ tickets) {
+    const T s = sent_val((T*)nullptr);
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        if (a) a[i] = s;
+        if (b) b[i] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) tickets[threadIdx.x] = 0;
+}
+
+template <typename T, typename F>
+__global__ void k_gather_pad(int64_t n, const int* __restrict__ map, const T* __restrict__ src, F* __restrict__ dst);
+
+template <>
+__global__ void k_gather_pad<double, double>(int64_t n, const int* __restrict__ map, const double* __restrict__ src,
+                                             double* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBS) {
+        const int m = map[e];
+        dst[e] = m >= 0 ? src[m] : 0.0;
+    }
+}
+template <>
+__global__ void k_gather_pad<double, float>(int64_t n, const int* __restrict__ map, const double* __restrict__ src,
+                                            float* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBS) {
+        const int m = map[e];
+        dst[e] = m >= 0 ? (float)src[m] : 0.0f;
+    }
+}
+template <>
+__global__ void k_gather_pad<zc, zc>(int64_t n, const int* __restrict__ map, const zc* __restrict__ src,
+                                     zc* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBS) {
+        const int m = map[e];
+        dst[e] = m >= 0 ? src[m] : zc{0.0, 0.0};
+    }
+}
+template <>
+__global__ void k_gather_pad<zc, fc>(int64_t n, const int* __restrict__ map, const zc* __restrict__ src,
+                                     fc* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBS) {
+        const int m = map[e];
+        dst[e] = m >= 0 ? fc{(float)src[m].re, (float)src[m].im} : fc{0.0f, 0.0f};
+    }
+}
+
+inline int grid_for(int64_t n) {
+    int64_t g = (n + kBS - 1) / kBS;
+    if (g < 1) g = 1;
+    if (g > 4096) g = 4096;
+    return (int)g;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
+    hipLaunchKernelGGL(k_sf_prepare<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, tickets);
+}
+
+template <typename T, typename F>
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int reverse, int* ticket, int* err,
+                     const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
+                     const T* rhs, const T* aux, const T* dd, T* x) {
+    const int nchunks = (n + kSfRows - 1) / kSfRows;
+#define SSGPU_SF(MODE, K)                                                                                   \
+    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(nchunks), dim3(kSfRows), 0, s, n, nchunks, reverse, \
+                       ticket, err, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+#define SSGPU_SF_K(MODE)                  \
+    if (kmax <= 16) SSGPU_SF(MODE, 16);   \
+    else if (kmax <= 32) SSGPU_SF(MODE, 32); \
+    else SSGPU_SF(MODE, 64);
+    if (mode == FWD_DIV_LAST) { SSGPU_SF_K(FWD_DIV_LAST) }
+    else if (mode == BWD_IC) { SSGPU_SF_K(BWD_IC) }
+    else { SSGPU_SF_K(BWD_DIV_FIRST) }
+#undef SSGPU_SF_K
+#undef SSGPU_SF
+}
+
+template <typename T, typename F>
+void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst) {
+    hipLaunchKernelGGL((k_gather_pad<T, F>), dim3(grid_for(n)), dim3(kBS), 0, s, n, map, src, dst);
+}
+
+template void launch_sf_prepare<double>(hipStream_t, int, double*, double*, int*);
+template void launch_sf_prepare<zc>(hipStream_t, int, zc*, zc*, int*);
+template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                              const int*, const double*, const double*, const double*,
+                                              const double*, const double*, double*);
+template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                             const int*, const float*, const float*, const double*, const double*,
+                                             const double*, double*);
+template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                      const int*, const zc*, const zc*, const zc*, const zc*, const zc*, zc*);
+template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                      const int*, const fc*, const fc*, const zc*, const zc*, const zc*, zc*);
+template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
+template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
+template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);
+template void launch_gather_pad<zc, fc>(hipStream_t, int64_t, const int*, const zc*, fc*);
+
+}  // namespace ssgpu

@@ -265,6 +265,21 @@ int32_t ssv_edge_unknown(const ssv_params* p, int i, int j, int k, int l) {
     return g.id(d, ei, ej, ek);
 }
 
+int ssv_unknown_edges(const ssv_params* p, int32_t* dir, int32_t* i, int32_t* j, int32_t* k) {
+    if (!valid(p) || !dir || !i || !j || !k) return 1;
+    Grid g;
+    build_grid(p, g);
+    std::vector<RowEdge> rows;
+    row_edges(g, rows);
+    for (int32_t r = 0; r < g.n; ++r) {
+        dir[r] = rows[r].d;
+        i[r] = rows[r].i;
+        j[r] = rows[r].j;
+        k[r] = rows[r].k;
+    }
+    return 0;
+}
+
 int ssv_count(const ssv_params* p, int32_t* n_unknowns, int64_t* nnz) {
     if (!valid(p)) return 1;
     Grid g;

@@ -50,6 +50,7 @@ def _lib():
         lib.ssv_element_matrices.argtypes = [ctypes.c_double] + [ctypes.c_void_p] * 3
         lib.ssv_edge_unknown.argtypes = [ctypes.POINTER(_Params)] + [ctypes.c_int] * 4
         lib.ssv_edge_unknown.restype = ctypes.c_int32
+        lib.ssv_unknown_edges.argtypes = [ctypes.POINTER(_Params)] + [ctypes.c_void_p] * 4
         _LIB = lib
     return _LIB
 
@@ -187,3 +188,26 @@ def edge_unknown(params, i, j, k, l):
     r = _lib().ssv_edge_unknown(ctypes.byref(cp), i, j, k, l)
     del keep
     return r
+
+
+def unknown_edges(params):
+    """(dir, i, j, k) start node of every unknown's edge, int32 arrays."""
+    cp, keep = _c_params(params)
+    n = ctypes.c_int32()
+    nnz = ctypes.c_int64()
+    _lib().ssv_count(ctypes.byref(cp), ctypes.byref(n), ctypes.byref(nnz))
+    out = [np.empty(n.value, np.int32) for _ in range(4)]
+    rc = _lib().ssv_unknown_edges(ctypes.byref(cp), *[a.ctypes.data for a in out])
+    del keep
+    if rc != 0:
+        raise ValueError("bad voxel parameters")
+    return tuple(out)
+
+
+def gradient(params, phi):
+    """Edge DOFs (line integrals) of grad(phi) for nodal phi[(nx+1),(ny+1),(nz+1)] indexed [i,j,k]."""
+    d, i, j, k = unknown_edges(params)
+    di = (d == 0).astype(np.int32)
+    dj = (d == 1).astype(np.int32)
+    dk = (d == 2).astype(np.int32)
+    return phi[i + di, j + dj, k + dk] - phi[i, j, k]
