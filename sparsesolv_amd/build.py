@@ -2,6 +2,8 @@
 
   libsparsesolv_amd.so  HIP kernels + solver drivers + C-ABI  (hipcc, gfx950)
   libssvoxel.so         curl-curl voxel generator            (g++, OpenMP)
+  libSparseSolv.so      SparseMat / MatSolverICCG / MatSolverMRTR C++ classes (g++)
+  SparseSolvPy*.so      pybind11 module with the reference's Python names
 
 Run as `python -m sparsesolv_amd.build` or through __graft_entry__.build().
 """
@@ -35,7 +37,7 @@ def _run(cmd):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_capi.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_capi.hip")]
     out = os.path.join(LIB, "libsparsesolv_amd.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
@@ -52,10 +54,37 @@ def build_voxel():
     return out
 
 
+def build_cpp():
+    """libSparseSolv.so: the reference's C++ classes over the C-ABI."""
+    out = os.path.join(LIB, "libSparseSolv.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", f"-I{INC}", os.path.join(CSRC, "sparsesolv_cpp.cpp"),
+          f"-L{LIB}", "-lsparsesolv_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_pybind():
+    """SparseSolvPy extension module (pybind11) over libSparseSolv.so."""
+    import sysconfig
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(LIB, "SparseSolvPy" + suffix)
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", f"-I{INC}",
+          f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+          os.path.join(CSRC, "pybind_module.cpp"), f"-L{LIB}", "-lSparseSolv", "-lsparsesolv_amd",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
     os.makedirs(LIB, exist_ok=True)
     build_voxel()
     build_hip()
+    build_cpp()
+    build_pybind()
 
 
 if __name__ == "__main__":

@@ -14,6 +14,7 @@
 // packet order).
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -92,13 +93,15 @@ struct LevelCsr {
 };
 
 // Sliced-ELL copy of a triangular structure for the sync-free sweeps: rows in
-// the matrix's own order, 64-row slices padded to the slice's longest row,
-// entry k of row (64*s + l) at slice_off[s] + 64*k + l.  Off-diagonal entries
-// only, in summation order; the diagonal is kept per row.  Two value sources:
-// the level-ordered factor (f) and the matrix values (a).
+// level order (position p holds row srow[p]), 64-position slices padded to the
+// slice's longest row, entry k of position (64*s + l) at slice_off[s] + 64*k + l.
+// Off-diagonal entries only, in summation order; the diagonal is kept per
+// position.  Two value sources: the level-ordered factor (f) and the matrix
+// values (a).
 struct SfCsr {
     int kmax = 0;          // longest row; 0 when a row exceeds 64 entries (level path used)
     int64_t nell = 0;
+    const int* srow = nullptr;   // level position -> row (the LevelCsr's prow)
     DevBuf slice_off, rlen, ecol, emap_f, dmap_f, emap_a, dmap_a;
     DevBuf eval, dval, eval32, dval32;
 };
@@ -117,6 +120,12 @@ struct Matrix {
     LevelCsr L, Uic, Us;
     DevBuf ppos;   // original row -> position in L
     SfCsr sfL, sfUic, sfUs;   // sync-free sweep copies of lower(A), transpose(lower), upper(A)
+    // IC(0) update lists (built on the first factorization)
+    int lmax_row = 0;          // longest row of lower(A)
+    bool icsym_built = false;
+    int64_t icsym_total = 0;
+    DevBuf ic_po, ic_slot, ic_pos, ic_col;
+    int sweep_grid = 512;     // persistent one-wave workgroups per sweep
     bool uic_built = false, us_built = false;
     // factor / preconditioner values and work space (grown on demand)
     DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;
@@ -152,7 +161,7 @@ static std::vector<int> levels(int n, const std::vector<int>& rp, const std::vec
 // Build a LevelCsr from a host CSR (rp, ci) plus per-entry source map.
 static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const std::vector<int>& ci,
                             const std::vector<int>& src, bool forward, LevelCsr& out,
-                            std::vector<int>* pos_out) {
+                            std::vector<int>* pos_out, std::vector<int>* order_out = nullptr) {
     const std::vector<int> lev = levels(n, rp, ci, forward);
     int nlev = 0;
     for (int i = 0; i < n; ++i) nlev = std::max(nlev, lev[i] + 1);
@@ -181,6 +190,7 @@ static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const
     }
     out.nlev = nlev;
     out.nnz = e;
+    if (order_out) *order_out = order;
     upload(c, out.prp, prp);
     upload(c, out.pci, pci);
     upload(c, out.prow, prow);
@@ -192,33 +202,35 @@ struct SfRow {
     std::vector<int> col, sf, sa;
     int df = -1, da = -1;
 };
-static void build_sf(Context* c, int n, const std::vector<SfRow>& rows, SfCsr& out) {
+static void build_sf(Context* c, int n, const std::vector<int>& order, const std::vector<SfRow>& rows,
+                     const LevelCsr& lv, SfCsr& out) {
     int kmax = 0;
     for (const auto& r : rows) kmax = std::max(kmax, (int)r.col.size());
     const int nsl = (n + 63) / 64;
     std::vector<int> soff(nsl + 1, 0), rlen(n);
-    for (int s = 0; s < nsl; ++s) {
+    for (int sl = 0; sl < nsl; ++sl) {
         int w = 0;
-        for (int i = 64 * s; i < std::min(n, 64 * s + 64); ++i) w = std::max(w, (int)rows[i].col.size());
-        soff[s + 1] = soff[s] + 64 * w;
+        for (int p = 64 * sl; p < std::min(n, 64 * sl + 64); ++p) w = std::max(w, (int)rows[order[p]].col.size());
+        soff[sl + 1] = soff[sl] + 64 * w;
     }
     const int64_t nell = soff[nsl];
     std::vector<int> ecol(std::max<int64_t>(nell, 1), 0), ef(std::max<int64_t>(nell, 1), -1),
         ea(std::max<int64_t>(nell, 1), -1), df(n), da(n);
-    for (int i = 0; i < n; ++i) {
-        const SfRow& r = rows[i];
-        rlen[i] = (int)r.col.size();
-        const int base = soff[i / 64] + (i % 64);
+    for (int p = 0; p < n; ++p) {
+        const SfRow& r = rows[order[p]];
+        rlen[p] = (int)r.col.size();
+        const int base = soff[p / 64] + (p % 64);
         for (size_t k = 0; k < r.col.size(); ++k) {
             ecol[base + 64 * k] = r.col[k];
             ef[base + 64 * k] = r.sf[k];
             ea[base + 64 * k] = r.sa[k];
         }
-        df[i] = r.df;
-        da[i] = r.da;
+        df[p] = r.df;
+        da[p] = r.da;
     }
     out.kmax = kmax <= 64 ? std::max(kmax, 1) : 0;
     out.nell = nell;
+    out.srow = lv.prow.as<int>();
     upload(c, out.slice_off, soff);
     upload(c, out.rlen, rlen);
     upload(c, out.ecol, ecol);
@@ -246,6 +258,7 @@ static void analyse(Matrix& m) {
     }
     m.nblk = (int)blk.size() - 1;
     upload(c, m.blk, blk);
+    if (const char* g = std::getenv("SSGPU_SWEEP_GRID")) m.sweep_grid = std::max(1, std::atoi(g));
     std::vector<int> diag(n, -1);
     for (int i = 0; i < n; ++i)
         for (int k = rp[i]; k < rp[i + 1]; ++k)
@@ -262,8 +275,10 @@ static void analyse(Matrix& m) {
         lrp[i + 1] = (int)lci.size();
         if (lrp[i + 1] == lrp[i] || lci[lrp[i + 1] - 1] != i) m.lower_diag_ok = false;
     }
-    std::vector<int> pos;
-    build_level_csr(c, n, lrp, lci, lsrc, true, m.L, &pos);
+    m.lmax_row = 0;
+    for (int i = 0; i < n; ++i) m.lmax_row = std::max(m.lmax_row, lrp[i + 1] - lrp[i]);
+    std::vector<int> pos, order_l, order_u, order_s;
+    build_level_csr(c, n, lrp, lci, lsrc, true, m.L, &pos, &order_l);
     upload(c, m.ppos, pos);
     // transpose(lower) (SprsTrans.cpp:312-319): row j holds (j, i) for lower entries (i, j),
     // ascending i; its source is the entry's position in the level-ordered L.
@@ -290,7 +305,7 @@ static void analyse(Matrix& m) {
                 uci[q] = i;
                 usrc[q] = lstart[i] + (k - lrp[i]);
             }
-        build_level_csr(c, n, urp, uci, usrc, false, m.Uic, nullptr);
+        build_level_csr(c, n, urp, uci, usrc, false, m.Uic, nullptr, &order_u);
         m.uic_built = true;
         // sync-free copies: lower(A) (off-diagonals, diagonal = last entry) and transpose(lower)
         std::vector<SfRow> rl(n), ru(n);
@@ -308,8 +323,8 @@ static void analyse(Matrix& m) {
                 }
             }
         }
-        build_sf(c, n, rl, m.sfL);
-        build_sf(c, n, ru, m.sfUic);
+        build_sf(c, n, order_l, rl, m.L, m.sfL);
+        build_sf(c, n, order_u, ru, m.Uic, m.sfUic);
     }
     // upper(A): entries col >= row (SprsTrans.cpp:387-394), for the SGS sweeps
     {
@@ -321,7 +336,7 @@ static void analyse(Matrix& m) {
             urp[i + 1] = (int)uci.size();
             if (urp[i + 1] == urp[i] || uci[urp[i]] != i) m.upper_diag_ok = false;
         }
-        build_level_csr(c, n, urp, uci, usrc, false, m.Us, nullptr);
+        build_level_csr(c, n, urp, uci, usrc, false, m.Us, nullptr, &order_s);
         m.us_built = true;
         std::vector<SfRow> rs(n);
         for (int i = 0; i < n; ++i)
@@ -329,7 +344,7 @@ static void analyse(Matrix& m) {
                 if (k == urp[i] && uci[k] == i) { rs[i].da = usrc[k]; continue; }
                 rs[i].col.push_back(uci[k]); rs[i].sf.push_back(-1); rs[i].sa.push_back(usrc[k]);
             }
-        build_sf(c, n, rs, m.sfUs);
+        build_sf(c, n, order_s, rs, m.Us, m.sfUs);
     }
 }
 
@@ -420,7 +435,8 @@ struct Solver {
     template <typename T, typename F>
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
                   const T* aux, const T* dd, T* x) {
-        launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, reverse ? 1 : 0, ticket, errflag(), S.slice_off.as<int>(),
+        (void)reverse;   // backward structures are already in their own level order
+        launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), S.srow, S.slice_off.as<int>(),
                               S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd, x);
     }
     // values of a sliced-ELL copy from a source array (factor or matrix values)
@@ -534,6 +550,35 @@ struct Solver {
     }
 
     // --- shifted IC(0) ------------------------------------------------------
+    // Update lists of the factorization (once per matrix); false if they would
+    // not fit 32-bit offsets (the searching level kernel is used then).
+    bool ic_symbolic() {
+        if (m.icsym_built) return m.icsym_total >= 0;
+        m.icsym_built = true;
+        const int64_t nl = m.L.nnz;
+        // every entry has fewer updates than its row has entries, so
+        // nnzL * longest row bounds the list length; keep it in 32-bit offsets
+        if ((nl + 1) * (int64_t)std::max(m.lmax_row, 1) > 2147483647LL) { m.icsym_total = -1; return false; }
+        DevBuf cnt, tmp;
+        cnt.alloc(c, sizeof(int) * (nl + 1));
+        m.ic_po.alloc(c, sizeof(int) * (nl + 1));
+        launch_fill<int>(s, nl + 1, cnt.as<int>(), 0);
+        launch_ic_symbolic_count(s, m.n, m.L.prp.as<int>(), m.L.pci.as<int>(), m.L.prow.as<int>(),
+                                 m.ppos.as<int>(), cnt.as<int>());
+        const size_t tb = scan_tmp_bytes((int)(nl + 1));
+        tmp.alloc(c, std::max<size_t>(tb, 8));
+        launch_exclusive_scan(s, cnt.as<int>(), m.ic_po.as<int>(), (int)(nl + 1), tmp.p, tb);
+        int total = 0;
+        SSGPU_HIP(hipMemcpyAsync(&total, m.ic_po.as<int>() + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+        SSGPU_HIP(hipStreamSynchronize(s));
+        m.icsym_total = total;
+        m.ic_slot.alloc(c, std::max<size_t>((size_t)total, 1));
+        m.ic_pos.alloc(c, sizeof(int) * std::max<size_t>((size_t)total, 1));
+        m.ic_col.alloc(c, sizeof(int) * std::max<size_t>((size_t)total, 1));
+        launch_ic_symbolic_fill(s, m.n, m.L.prp.as<int>(), m.L.pci.as<int>(), m.L.prow.as<int>(), m.ppos.as<int>(),
+                                m.ic_po.as<int>(), m.ic_slot.p, m.ic_pos.as<int>(), m.ic_col.as<int>());
+        return true;
+    }
     // Factor the matrix with values av (A or the scaled copy); returns pivot fixes.
     template <typename T> int factor(const T* av, double accel) {
         const int64_t nl = m.L.nnz;
@@ -544,10 +589,18 @@ struct Solver {
         launch_gather<T, T>(s, nl, m.L.map.as<int>(), av, m.Lv.as<T>());
         launch_fill<T>(s, m.n, m.diagD.as<T>(), Host<T>::zero());
         launch_fill<int>(s, 1, c->d_count.as<int>(), 0);
-        for (int l = 0; l < m.L.nlev; ++l)
-            launch_ic_level<T>(s, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(), m.L.pci.as<int>(),
-                               m.Lv.as<T>(), m.L.prow.as<int>(), m.ppos.as<int>(), m.diagD.as<T>(), accel,
-                               c->d_count.as<int>());
+        if (m.lmax_row <= 48 && ic_symbolic()) {
+            for (int l = 0; l < m.L.nlev; ++l)
+                launch_ic_numeric<T>(s, m.lmax_row, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(),
+                                     m.L.pci.as<int>(), m.L.prow.as<int>(), m.ic_po.as<int>(), m.ic_slot.p,
+                                     m.ic_pos.as<int>(), m.ic_col.as<int>(), m.Lv.as<T>(), m.diagD.as<T>(), accel,
+                                     c->d_count.as<int>());
+        } else {
+            for (int l = 0; l < m.L.nlev; ++l)
+                launch_ic_level<T>(s, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(), m.L.pci.as<int>(),
+                                   m.Lv.as<T>(), m.L.prow.as<int>(), m.ppos.as<int>(), m.diagD.as<T>(), accel,
+                                   c->d_count.as<int>());
+        }
         launch_gather<T, T>(s, m.Uic.nnz, m.Uic.map.as<int>(), m.Lv.as<T>(), m.Uv.as<T>());
         if (sf_ok()) {
             const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;

@@ -39,6 +39,21 @@ template <typename T>
 void launch_ic_level(hipStream_t s, int p0, int p1, const int* prp, const int* pci, T* pv, const int* prow,
                      const int* ppos, T* diagD, double accel, int* nfix);
 
+// --- shifted IC(0) with a precomputed update list (ssgpu_icfact.hip) -----------------
+// per factor entry, the number of updates (cnt) / the updates themselves
+// (slot of L_ik in its row, position of L_jk, column k) in the reference's order
+void launch_ic_symbolic_count(hipStream_t s, int n, const int* prp, const int* pci, const int* prow, const int* ppos,
+                              int* cnt);
+void launch_ic_symbolic_fill(hipStream_t s, int n, const int* prp, const int* pci, const int* prow, const int* ppos,
+                             const int* po, void* u_slot, int* u_pos, int* u_col);
+size_t scan_tmp_bytes(int nitems);
+void launch_exclusive_scan(hipStream_t s, const int* in, int* out, int nitems, void* tmp, size_t tmp_bytes);
+// one level; kmax_row >= longest row of lower(A) (<= 48)
+template <typename T>
+void launch_ic_numeric(hipStream_t s, int kmax_row, int p0, int p1, const int* prp, const int* pci, const int* prow,
+                       const int* po, const void* u_slot, const int* u_pos, const int* u_col, T* fv, T* diagD,
+                       double accel, int* nfix);
+
 // --- reductions -----------------------------------------------------------------
 // Grid size used for n elements (fixed function of n).
 int reduce_blocks(int64_t n);
@@ -99,12 +114,13 @@ template <typename T> void launch_mul_scale(hipStream_t s, int n, const T* sc, T
 template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d, int* count);
 
 // --- sync-free sweeps (ssgpu_sweep.hip) -------------------------------------------
-constexpr int kSfRows = 256;   // rows per chunk = threads per workgroup
+constexpr int kSfRows = 64;    // rows per chunk = threads per workgroup (one wave)
 // fill a and b (either may be null) with the not-ready sentinel; zero two tickets
 template <typename T> void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets);
-// one sweep over all rows; kmax >= longest row (off-diagonals), <= 64
+// one sweep over all rows in level order (srow: level position -> row); kmax >=
+// longest row (off-diagonals), <= 64; grid = persistent one-wave workgroups
 template <typename T, typename F>
-void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int reverse, int* ticket, int* err,
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, T* x);
 // dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)

@@ -1,25 +1,27 @@
-// Sync-free level-free triangular sweeps for gfx950.
+// Sync-free triangular sweeps for gfx950.
 //
-// One launch per sweep.  Rows are cut into chunks of 256 consecutive rows
-// (in the matrix's own order); a 256-thread workgroup takes a ticket from an
-// atomic counter and owns the chunk with that ticket (ascending chunks for a
-// forward sweep, descending for a backward one), one row per lane.  Because
-// tickets are handed out in dependency order, every chunk a workgroup waits
-// on belongs to a workgroup that is already running, so the sweep cannot
-// deadlock however the hardware dispatches workgroups.
+// One launch per sweep.  Rows are taken in level order (a row's level is one
+// more than the highest level among the rows it reads) and cut into chunks of
+// 64 rows, one lane per row.  A persistent grid of one-wave workgroups takes
+// chunk tickets from an atomic counter in that order.  Every input of a row
+// lies at a lower level, hence in a chunk with a smaller ticket (or in the same
+// chunk, handled by the same wave), and a ticket is only taken by a running
+// workgroup: the sweep cannot deadlock however workgroups are dispatched, and
+// the chunks in flight are the ones at the level front, so the grid size
+// bounds the polling traffic.
 //
 // A lane keeps its row's off-diagonal entries in registers (loaded from a
-// sliced-ELL copy, 64 rows per slice, entry k of lane l at slice_base +
-// 64*k + l, so each load instruction is one coalesced 512-byte access) and
-// resolves its inputs as they appear: rows of its own chunk through LDS,
-// rows of other chunks through agent-scope atomic loads of the output
-// vector, which the prepare kernel fills with a NaN sentinel (the value is
-// its own ready flag; a computed value never has this bit pattern because
-// arithmetic only produces quiet NaNs).  Once every input is present the lane
-// forms the row's result with the reference's sequential summation order and
-// publishes it to LDS and to global memory.  A spin limit bounds every wait:
-// a lane that exceeds it sets the error flag and leaves, so the grid always
+// sliced-ELL copy in level order, entry k of slot l of chunk c at
+// slice_off[c] + 64*k + l: each load instruction is one coalesced 512-byte
+// access).  Readiness needs no flags: the output vector is pre-filled with a
+// NaN sentinel, results are published with agent-scope stores, and each pass
+// a lane reloads, in one batch of agent-scope loads, the inputs it still
+// misses (arithmetic only produces quiet NaNs, never this bit pattern).  The row's result is then formed in the
+// reference's sequential summation order.  A spin limit bounds every wait: a
+// wave that exceeds it sets the error flag and leaves, so the grid always
 // drains.
+#include <algorithm>
+
 #include "ssgpu_kernels.h"
 
 namespace ssgpu {
@@ -54,24 +56,6 @@ __device__ inline void gstore(zc* p, zc v) {
     gstore(d, v.re);
     gstore(d + 1, v.im);
 }
-// Plain LDS reads: the spin loop starts every iteration with a compiler
-// memory barrier, so values are re-read each time, and an aligned 8-byte LDS
-// access is single-copy atomic on CDNA.
-__device__ inline double lload(const double* p) { return *p; }
-__device__ inline zc lload(const zc* p) {
-    const double* d = reinterpret_cast<const double*>(p);
-    return zc{lload(d), lload(d + 1)};
-}
-__device__ inline void lstore(double* p, double v) {
-    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ inline void lstore(zc* p, zc v) {
-    double* d = reinterpret_cast<double*>(p);
-    lstore(d, v.re);
-    lstore(d + 1, v.im);
-}
-
 template <typename T, typename F> __device__ inline T wid(F a);
 template <> __device__ inline double wid<double, double>(double a) { return a; }
 template <> __device__ inline double wid<double, float>(float a) { return (double)a; }
@@ -79,75 +63,67 @@ template <> __device__ inline zc wid<zc, zc>(zc a) { return a; }
 template <> __device__ inline zc wid<zc, fc>(fc a) { return zc{(double)a.re, (double)a.im}; }
 
 template <typename T, typename F, int MODE, int KMAX>
-__global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int reverse, int* __restrict__ ticket,
-                                                      int* __restrict__ err, const int* __restrict__ slice_off,
-                                                      const int* __restrict__ rlen, const int* __restrict__ ecol,
-                                                      const F* __restrict__ eval, const F* __restrict__ dval,
-                                                      const T* __restrict__ rhs, const T* __restrict__ aux,
-                                                      const T* __restrict__ dd, T* __restrict__ x) {
-    __shared__ T tl[kSfRows];
+__global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* __restrict__ ticket,
+                                                      int* __restrict__ err, const int* __restrict__ srow,
+                                                      const int* __restrict__ slice_off, const int* __restrict__ rlen,
+                                                      const int* __restrict__ ecol, const F* __restrict__ eval,
+                                                      const F* __restrict__ dval, const T* __restrict__ rhs,
+                                                      const T* __restrict__ aux, const T* __restrict__ dd,
+                                                      T* __restrict__ x) {
     __shared__ int s_chunk;
-    if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1);
-    tl[threadIdx.x] = sent_val((T*)nullptr);
-    __syncthreads();
-    const int t = s_chunk;
-    if (t >= nchunks) return;
-    const int c = reverse ? nchunks - 1 - t : t;
-    const int c0 = c * kSfRows;
-    const int i = c0 + (int)threadIdx.x;
-    const bool active = i < n;
-    const int lane = threadIdx.x & 63;
-    int len = 0;
-    int cols[KMAX];
-    F vals[KMAX];
-    T g[KMAX];
-    T r_i = Num<T>::zero(), a_i = Num<T>::zero(), d_i = Num<T>::zero();
-    F dg = F();
-    if (active) {
-        len = rlen[i];
-        const int base = slice_off[i >> 6] + lane;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < len) {
-                cols[k] = ecol[base + 64 * k];
-                vals[k] = eval[base + 64 * k];
-            }
-        if (MODE == BWD_IC) {
-            a_i = aux[i];
-            d_i = dd[i];
-        } else {
-            r_i = rhs[i];
-            dg = dval[i];
-        }
-    }
-    unsigned long long miss = active ? (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) : 0ull;
-    bool done = !active;
+    const int lane = threadIdx.x;
     long spins = 0;
     while (true) {
-        __asm__ __volatile__("" ::: "memory");
-        bool progress = false;
-        if (!done) {
-            // one generic (flat) load per missing input, pointing into LDS for
-            // rows of this chunk and into global memory otherwise: no branches,
-            // so all loads are in flight together before the single wait
-            T v[KMAX];
-            const unsigned long long want = miss;
+        if (lane == 0) s_chunk = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int c = s_chunk;
+        __syncthreads();
+        if (c >= nchunks) return;
+        const int p = c * kSfRows + lane;   // position in level order
+        const bool active = p < n;
+        const int i = active ? srow[p] : 0;
+        int len = 0;
+        int cols[KMAX];
+        F vals[KMAX];
+        T g[KMAX];
+        T r_i = Num<T>::zero(), a_i = Num<T>::zero(), d_i = Num<T>::zero();
+        F dg = F();
+        if (active) {
+            len = rlen[p];
+            const int base = slice_off[c] + lane;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (want & (1ull << k)) {
-                    const int jl = cols[k] - c0;
-                    const T* p = (jl >= 0 && jl < kSfRows) ? (const T*)&tl[jl] : x + cols[k];
-                    v[k] = gload(p);
+            for (int k = 0; k < KMAX; ++k)
+                if (k < len) {
+                    cols[k] = ecol[base + kSfRows * k];
+                    vals[k] = eval[base + kSfRows * k];
                 }
+            if (MODE == BWD_IC) {
+                a_i = aux[i];
+                d_i = dd[i];
+            } else {
+                r_i = rhs[i];
+                dg = dval[p];
             }
+        }
+        unsigned long long miss = active ? (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) : 0ull;
+        bool done = !active;
+        int idle = 0;
+        while (true) {
+            __asm__ __volatile__("" ::: "memory");
+            bool progress = false;
+            if (!done && miss) {
+                // fetch every input still missing in one batch (after the first
+                // pass these are only the few rows of the preceding levels)
+                const unsigned long long want = miss;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if ((want & (1ull << k)) && !sent(v[k])) {
-                    g[k] = v[k];
-                    miss &= ~(1ull << k);
-                }
+                for (int k = 0; k < KMAX; ++k)
+                    if (want & (1ull << k)) g[k] = gload(x + cols[k]);
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if ((want & (1ull << k)) && !sent(g[k])) miss &= ~(1ull << k);
+                progress = miss != want;
             }
-            if (miss == 0ull) {
+            if (!done && miss == 0ull) {
                 T s = (MODE == BWD_IC) ? Num<T>::zero() : r_i;
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
@@ -155,18 +131,24 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int re
                 T out;
                 if (MODE == BWD_IC) out = Num<T>::add(Num<T>::mul(s, d_i), a_i);
                 else out = Num<T>::div(s, wid<T, F>(dg));
-                lstore(&tl[threadIdx.x], out);
                 gstore(x + i, out);
                 done = true;
                 progress = true;
             }
+            if (__all(done)) break;
+            if (++spins > kSpinLimit) {
+                atomicOr(err, 1);
+                return;
+            }
+            if (__any(progress)) {
+                idle = 0;
+            } else {
+                // back off while nothing arrives: 64, then 128 clocks
+                if (idle < 2) ++idle;
+                if (idle == 1) __builtin_amdgcn_s_sleep(1);
+                else __builtin_amdgcn_s_sleep(2);
+            }
         }
-        if (__all(done)) break;
-        if (++spins > kSpinLimit) {
-            atomicOr(err, 1);
-            break;
-        }
-        if (!__any(progress)) __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -231,15 +213,16 @@ void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
 }
 
 template <typename T, typename F>
-void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int reverse, int* ticket, int* err,
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, T* x) {
     const int nchunks = (n + kSfRows - 1) / kSfRows;
-#define SSGPU_SF(MODE, K)                                                                                   \
-    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(nchunks), dim3(kSfRows), 0, s, n, nchunks, reverse, \
-                       ticket, err, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
-#define SSGPU_SF_K(MODE)                  \
-    if (kmax <= 16) SSGPU_SF(MODE, 16);   \
+    const int g = std::max(1, std::min(grid, nchunks));
+#define SSGPU_SF(MODE, K)                                                                                      \
+    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err, srow, \
+                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+#define SSGPU_SF_K(MODE)                     \
+    if (kmax <= 16) SSGPU_SF(MODE, 16);      \
     else if (kmax <= 32) SSGPU_SF(MODE, 32); \
     else SSGPU_SF(MODE, 64);
     if (mode == FWD_DIV_LAST) { SSGPU_SF_K(FWD_DIV_LAST) }
@@ -256,16 +239,16 @@ void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F
 
 template void launch_sf_prepare<double>(hipStream_t, int, double*, double*, int*);
 template void launch_sf_prepare<zc>(hipStream_t, int, zc*, zc*, int*);
-template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
                                               const int*, const double*, const double*, const double*,
                                               const double*, const double*, double*);
-template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                             const int*, const float*, const float*, const double*, const double*,
+template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
+                                              const int*, const float*, const float*, const double*, const double*,
                                              const double*, double*);
-template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                      const int*, const zc*, const zc*, const zc*, const zc*, const zc*, zc*);
-template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                      const int*, const fc*, const fc*, const zc*, const zc*, const zc*, zc*);
+template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
+                                              const int*, const zc*, const zc*, const zc*, const zc*, const zc*, zc*);
+template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
+                                              const int*, const fc*, const fc*, const zc*, const zc*, const zc*, zc*);
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);
