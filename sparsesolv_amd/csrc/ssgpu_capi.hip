@@ -57,6 +57,8 @@ struct Context {
     std::mutex mu;
 };
 
+
+
 void DevBuf::alloc(Context* c, size_t b) {
     release();
     ctx = c;
@@ -125,7 +127,7 @@ struct Matrix {
     bool icsym_built = false;
     int64_t icsym_total = 0;
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
-    int sweep_grid = 512;     // persistent one-wave workgroups per sweep
+    int sweep_grid = 256;     // persistent one-wave workgroups per sweep
     bool uic_built = false, us_built = false;
     // factor / preconditioner values and work space (grown on demand)
     DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;
@@ -214,6 +216,7 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const std
         soff[sl + 1] = soff[sl] + 64 * w;
     }
     const int64_t nell = soff[nsl];
+
     std::vector<int> ecol(std::max<int64_t>(nell, 1), 0), ef(std::max<int64_t>(nell, 1), -1),
         ea(std::max<int64_t>(nell, 1), -1), df(n), da(n);
     for (int p = 0; p < n; ++p) {

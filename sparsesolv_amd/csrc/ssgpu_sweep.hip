@@ -7,19 +7,22 @@
 // lies at a lower level, hence in a chunk with a smaller ticket (or in the same
 // chunk, handled by the same wave), and a ticket is only taken by a running
 // workgroup: the sweep cannot deadlock however workgroups are dispatched, and
-// the chunks in flight are the ones at the level front, so the grid size
-// bounds the polling traffic.
+// the chunks in flight are the ones at the level front.
 //
 // A lane keeps its row's off-diagonal entries in registers (loaded from a
 // sliced-ELL copy in level order, entry k of slot l of chunk c at
 // slice_off[c] + 64*k + l: each load instruction is one coalesced 512-byte
 // access).  Readiness needs no flags: the output vector is pre-filled with a
-// NaN sentinel, results are published with agent-scope stores, and each pass
-// a lane reloads, in one batch of agent-scope loads, the inputs it still
-// misses (arithmetic only produces quiet NaNs, never this bit pattern).  The row's result is then formed in the
-// reference's sequential summation order.  A spin limit bounds every wait: a
-// wave that exceeds it sets the error flag and leaves, so the grid always
-// drains.
+// NaN sentinel (arithmetic only produces quiet NaNs, never this bit pattern),
+// results are published with agent-scope stores and read back with
+// agent-scope loads.  The row's result is then formed in the reference's
+// sequential summation order.  A spin limit bounds every wait: a wave that
+// exceeds it sets the error flag and leaves, so the grid always drains.
+//
+// The cost of a level is the latency of publishing a value and seeing it
+// (most of it the agent-scope load round trip) plus the length of one polling
+// pass, not bytes: a pass reloads only the inputs a lane still misses, and the
+// register bucket (row length rounded up) keeps the pass short.
 #include <algorithm>
 
 #include "ssgpu_kernels.h"
@@ -62,41 +65,29 @@ template <> __device__ inline double wid<double, float>(float a) { return (doubl
 template <> __device__ inline zc wid<zc, zc>(zc a) { return a; }
 template <> __device__ inline zc wid<zc, fc>(fc a) { return zc{(double)a.re, (double)a.im}; }
 
-template <typename T, typename F, int MODE, int KMAX>
-__global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* __restrict__ ticket,
-                                                      int* __restrict__ err, const int* __restrict__ srow,
-                                                      const int* __restrict__ slice_off, const int* __restrict__ rlen,
-                                                      const int* __restrict__ ecol, const F* __restrict__ eval,
-                                                      const F* __restrict__ dval, const T* __restrict__ rhs,
-                                                      const T* __restrict__ aux, const T* __restrict__ dd,
-                                                      T* __restrict__ x) {
-    __shared__ int s_chunk;
-    const int lane = threadIdx.x;
-    long spins = 0;
-    while (true) {
-        if (lane == 0) s_chunk = atomicAdd(ticket, 1);
-        __syncthreads();
-        const int c = s_chunk;
-        __syncthreads();
-        if (c >= nchunks) return;
-        const int p = c * kSfRows + lane;   // position in level order
-        const bool active = p < n;
-        const int i = active ? srow[p] : 0;
-        int len = 0;
-        int cols[KMAX];
-        F vals[KMAX];
-        T g[KMAX];
-        T r_i = Num<T>::zero(), a_i = Num<T>::zero(), d_i = Num<T>::zero();
-        F dg = F();
-        if (active) {
-            len = rlen[p];
-            const int base = slice_off[c] + lane;
+// Row state: the lane's row of the chunk, its ELL entries in registers.
+template <typename T, typename F, int MODE, int KMAX> struct RowState {
+    int i = 0, len = 0;
+    int cols[KMAX];
+    F vals[KMAX];
+    T r_i, a_i, d_i;
+    F dg;
+    __device__ void load(int p, bool active, int c, const int* __restrict__ srow, const int* __restrict__ slice_off,
+                         const int* __restrict__ rlen, const int* __restrict__ ecol, const F* __restrict__ eval,
+                         const F* __restrict__ dval, const T* __restrict__ rhs, const T* __restrict__ aux,
+                         const T* __restrict__ dd) {
+        r_i = a_i = d_i = Num<T>::zero();
+        dg = F();
+        i = active ? srow[p] : 0;
+        len = active ? rlen[p] : 0;
+        const int base = slice_off[c] + (int)threadIdx.x;
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k)
-                if (k < len) {
-                    cols[k] = ecol[base + kSfRows * k];
-                    vals[k] = eval[base + kSfRows * k];
-                }
+        for (int k = 0; k < KMAX; ++k)
+            if (k < len) {
+                cols[k] = ecol[base + kSfRows * k];
+                vals[k] = eval[base + kSfRows * k];
+            }
+        if (active) {
             if (MODE == BWD_IC) {
                 a_i = aux[i];
                 d_i = dd[i];
@@ -105,33 +96,63 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
                 dg = dval[p];
             }
         }
-        unsigned long long miss = active ? (len >= 64 ? ~0ull : ((1ull << len) - 1ull)) : 0ull;
+    }
+    // the row's result from its inputs g, in the reference's summation order
+    __device__ T result(const T (&g)[KMAX]) const {
+        T s = (MODE == BWD_IC) ? Num<T>::zero() : r_i;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            if (k < len) s = Num<T>::sub(s, Num<T>::mul(wid<T, F>(vals[k]), g[k]));
+        if (MODE == BWD_IC) return Num<T>::add(Num<T>::mul(s, d_i), a_i);
+        return Num<T>::div(s, wid<T, F>(dg));
+    }
+};
+
+__device__ inline int take_ticket(int* ticket) {
+    __shared__ int s_chunk;
+    if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int c = s_chunk;
+    __syncthreads();
+    return c;
+}
+
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* __restrict__ ticket,
+                                                      int* __restrict__ err, const int* __restrict__ srow,
+                                                      const int* __restrict__ slice_off, const int* __restrict__ rlen,
+                                                      const int* __restrict__ ecol, const F* __restrict__ eval,
+                                                      const F* __restrict__ dval, const T* __restrict__ rhs,
+                                                      const T* __restrict__ aux, const T* __restrict__ dd,
+                                                      T* __restrict__ x) {
+    const int lane = threadIdx.x;
+    long spins = 0;
+    while (true) {
+        const int c = take_ticket(ticket);
+        if (c >= nchunks) return;
+        const int p = c * kSfRows + lane;   // position in level order
+        const bool active = p < n;
+        RowState<T, F, MODE, KMAX> row;
+        row.load(p, active, c, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
+        T g[KMAX];
+        unsigned long long miss = active ? (row.len >= 64 ? ~0ull : ((1ull << row.len) - 1ull)) : 0ull;
         bool done = !active;
         int idle = 0;
         while (true) {
             __asm__ __volatile__("" ::: "memory");
             bool progress = false;
             if (!done && miss) {
-                // fetch every input still missing in one batch (after the first
-                // pass these are only the few rows of the preceding levels)
                 const unsigned long long want = miss;
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
-                    if (want & (1ull << k)) g[k] = gload(x + cols[k]);
+                    if (want & (1ull << k)) g[k] = gload(x + row.cols[k]);
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
                     if ((want & (1ull << k)) && !sent(g[k])) miss &= ~(1ull << k);
                 progress = miss != want;
             }
             if (!done && miss == 0ull) {
-                T s = (MODE == BWD_IC) ? Num<T>::zero() : r_i;
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (k < len) s = Num<T>::sub(s, Num<T>::mul(wid<T, F>(vals[k]), g[k]));
-                T out;
-                if (MODE == BWD_IC) out = Num<T>::add(Num<T>::mul(s, d_i), a_i);
-                else out = Num<T>::div(s, wid<T, F>(dg));
-                gstore(x + i, out);
+                gstore(x + row.i, row.result(g));
                 done = true;
                 progress = true;
             }
@@ -205,12 +226,26 @@ inline int grid_for(int64_t n) {
     return (int)g;
 }
 
-}  // namespace
-
-template <typename T>
-void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
-    hipLaunchKernelGGL(k_sf_prepare<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, tickets);
+// register buckets: the row length rounded up (fewer predicated loads per poll
+// pass shortens the pass, and the pass length is part of every level's cost)
+template <typename T, typename F, int MODE>
+void sweep_kmax(hipStream_t s, int kmax, int g, int n, int nchunks, int* ticket, int* err, const int* srow,
+                const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval, const T* rhs,
+                const T* aux, const T* dd, T* x) {
+#define SSGPU_SF(K)                                                                                               \
+    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err, srow, \
+                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+    if (kmax <= 8) SSGPU_SF(8);
+    else if (kmax <= 12) SSGPU_SF(12);
+    else if (kmax <= 16) SSGPU_SF(16);
+    else if (kmax <= 20) SSGPU_SF(20);
+    else if (kmax <= 24) SSGPU_SF(24);
+    else if (kmax <= 32) SSGPU_SF(32);
+    else SSGPU_SF(64);
+#undef SSGPU_SF
 }
+
+}  // namespace
 
 template <typename T, typename F>
 void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
@@ -218,18 +253,20 @@ void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ti
                      const T* rhs, const T* aux, const T* dd, T* x) {
     const int nchunks = (n + kSfRows - 1) / kSfRows;
     const int g = std::max(1, std::min(grid, nchunks));
-#define SSGPU_SF(MODE, K)                                                                                      \
-    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err, srow, \
-                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
-#define SSGPU_SF_K(MODE)                     \
-    if (kmax <= 16) SSGPU_SF(MODE, 16);      \
-    else if (kmax <= 32) SSGPU_SF(MODE, 32); \
-    else SSGPU_SF(MODE, 64);
-    if (mode == FWD_DIV_LAST) { SSGPU_SF_K(FWD_DIV_LAST) }
-    else if (mode == BWD_IC) { SSGPU_SF_K(BWD_IC) }
-    else { SSGPU_SF_K(BWD_DIV_FIRST) }
-#undef SSGPU_SF_K
-#undef SSGPU_SF
+    if (mode == FWD_DIV_LAST)
+        sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
+                                       eval, dval, rhs, aux, dd, x);
+    else if (mode == BWD_IC)
+        sweep_kmax<T, F, BWD_IC>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol, eval,
+                                 dval, rhs, aux, dd, x);
+    else
+        sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
+                                        eval, dval, rhs, aux, dd, x);
+}
+
+template <typename T>
+void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
+    hipLaunchKernelGGL(k_sf_prepare<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, tickets);
 }
 
 template <typename T, typename F>
@@ -239,16 +276,18 @@ void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F
 
 template void launch_sf_prepare<double>(hipStream_t, int, double*, double*, int*);
 template void launch_sf_prepare<zc>(hipStream_t, int, zc*, zc*, int*);
-template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
-                                              const int*, const double*, const double*, const double*,
-                                              const double*, const double*, double*);
-template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
-                                              const int*, const float*, const float*, const double*, const double*,
-                                             const double*, double*);
-template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
-                                              const int*, const zc*, const zc*, const zc*, const zc*, const zc*, zc*);
-template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*, const int*,
-                                              const int*, const fc*, const fc*, const zc*, const zc*, const zc*, zc*);
+template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                         const int*, const int*, const double*, const double*, const double*, const double*,
+                                         const double*, double*);
+template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                         const int*, const int*, const float*, const float*, const double*, const double*,
+                                         const double*, double*);
+template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                         const int*, const int*, const zc*, const zc*, const zc*, const zc*,
+                                         const zc*, zc*);
+template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
+                                         const int*, const int*, const fc*, const fc*, const zc*, const zc*,
+                                         const zc*, zc*);
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);
