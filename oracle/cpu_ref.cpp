@@ -302,7 +302,8 @@ inline double normv(int n, const cplx* a) {
 }
 
 // p.(A p) as the GPU's fused SpMV computes it: one partial per SpMV row block
-// (<= 256 rows, <= 4096 entries unless one long row), then the finalize tree.
+// (<= 128 rows, <= 4096 entries unless one long row; row j's term in thread
+// slot j of a 256-thread block tree), then the finalize tree.
 template <typename T> T dot_spmv_order(const Csr<T>& A, const T* p, const T* ap) {
     if (g_red_mode != 1) return dotv(A.n, p, ap);
     constexpr int K = sizeof(T) / sizeof(double);
@@ -310,7 +311,7 @@ template <typename T> T dot_spmv_order(const Csr<T>& A, const T* p, const T* ap)
     int r = 0;
     while (r < A.n) {
         int r1 = r + 1;
-        while (r1 < A.n && r1 - r < kTB && A.rp[r1 + 1] - A.rp[r] <= 4096) ++r1;
+        while (r1 < A.n && r1 - r < 128 && A.rp[r1 + 1] - A.rp[r] <= 4096) ++r1;
         double th[K][kTB] = {};
         for (int t = 0; t < kTB && r + t < r1; ++t) {
             const T prod = p[r + t] * ap[r + t];

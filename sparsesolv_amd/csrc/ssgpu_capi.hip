@@ -127,7 +127,8 @@ struct Matrix {
     bool icsym_built = false;
     int64_t icsym_total = 0;
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
-    int sweep_grid = 256;     // persistent one-wave workgroups per sweep
+    int sweep_grid = 128;     // persistent one-wave workgroups per sweep
+    int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
     bool uic_built = false, us_built = false;
     // factor / preconditioner values and work space (grown on demand)
     DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;
@@ -248,13 +249,13 @@ static void analyse(Matrix& m) {
     const int n = m.n;
     const auto& rp = m.h_rp;
     const auto& ci = m.h_ci;
-    // SpMV row blocks: <= 256 rows and <= kSpmvCap entries (or one long row)
+    // SpMV row blocks: <= kSpmvRows rows and <= kSpmvCap entries (or one long row)
     std::vector<int> blk{0};
     {
         int r = 0;
         while (r < n) {
             int r1 = r + 1;
-            while (r1 < n && r1 - r < kBS && rp[r1 + 1] - rp[r] <= kSpmvCap) ++r1;
+            while (r1 < n && r1 - r < kSpmvRows && rp[r1 + 1] - rp[r] <= kSpmvCap) ++r1;
             blk.push_back(r1);
             r = r1;
         }
@@ -262,6 +263,7 @@ static void analyse(Matrix& m) {
     m.nblk = (int)blk.size() - 1;
     upload(c, m.blk, blk);
     if (const char* g = std::getenv("SSGPU_SWEEP_GRID")) m.sweep_grid = std::max(1, std::atoi(g));
+    if (const char* g = std::getenv("SSGPU_SWEEP_IDLE")) m.sweep_idle = std::min(5, std::max(1, std::atoi(g)));
     std::vector<int> diag(n, -1);
     for (int i = 0; i < n; ++i)
         for (int k = rp[i]; k < rp[i + 1]; ++k)
@@ -440,7 +442,7 @@ struct Solver {
                   const T* aux, const T* dd, T* x) {
         (void)reverse;   // backward structures are already in their own level order
         launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), S.srow, S.slice_off.as<int>(),
-                              S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd, x);
+                              S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd, m.sweep_idle, x);
     }
     // values of a sliced-ELL copy from a source array (factor or matrix values)
     template <typename T> void sf_values(SfCsr& S, const T* src, bool from_factor, bool with_f32) {

@@ -13,6 +13,7 @@ namespace ssgpu {
 
 constexpr int kBS = 256;          // threads per block of the streaming kernels
 constexpr int kSpmvCap = 4096;    // products staged in LDS per SpMV block
+constexpr int kSpmvRows = 128;    // rows per SpMV block (two threads per row)
 constexpr int kMaxReduceBlocks = 1024;
 
 // --- SpMV --------------------------------------------------------------------
@@ -122,7 +123,7 @@ template <typename T> void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, i
 template <typename T, typename F>
 void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
-                     const T* rhs, const T* aux, const T* dd, T* x);
+                     const T* rhs, const T* aux, const T* dd, int max_idle, T* x);
 // dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)
 template <typename T, typename F>
 void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);

@@ -63,6 +63,14 @@ template <int K> __device__ inline void block_sum(double* acc) {
 }
 
 // ---------------------------------------------------------------- SpMV
+__device__ inline double ntload(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ inline zc ntload(const zc* p) {
+    const double* d = reinterpret_cast<const double*>(p);
+    return zc{__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1)};
+}
+__device__ inline double shfl_xor1(double v) { return __shfl_xor(v, 1); }
+__device__ inline zc shfl_xor1(zc v) { return zc{__shfl_xor(v.re, 1), __shfl_xor(v.im, 1)}; }
+
 template <typename T>
 __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const T* __restrict__ v,
@@ -73,26 +81,49 @@ __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const
     const int r0 = blk[b], r1 = blk[b + 1];
     const int e0 = rp[r0], e1 = rp[r1];
     const int ne = e1 - e0;
-    const int r = r0 + (int)threadIdx.x;
+    // two threads per row: the even- and odd-position chains of the
+    // reference's two alternating accumulators, added at the end (ta + tb)
+    const int j = (int)threadIdx.x >> 1, par = (int)threadIdx.x & 1;
+    const int r = r0 + j;
     T yv = Num<T>::zero();
     if (ne <= kSpmvCap) {
-        for (int k = threadIdx.x; k < ne; k += kBS) prod[k] = Num<T>::mul(v[e0 + k], x[ci[e0 + k]]);
+        // products of the block's entries into LDS; a thread's kSpmvCap / kBS
+        // entries are loaded together (indices and values, then the gathers)
+        // so their latencies overlap
+        constexpr int U = kSpmvCap / kBS;
+        int cc[U];
+        T vv[U], xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = u * kBS + (int)threadIdx.x;
+            if (k < ne) {   // the matrix streams through once: keep it out of the caches' way
+                cc[u] = __builtin_nontemporal_load(ci + e0 + k);
+                vv[u] = ntload(v + e0 + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u * kBS + (int)threadIdx.x < ne) xx[u] = x[cc[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = u * kBS + (int)threadIdx.x;
+            if (k < ne) prod[k] = Num<T>::mul(vv[u], xx[u]);
+        }
         __syncthreads();
+        T acc = Num<T>::zero();
         if (r < r1) {
             const int e = rp[r + 1] - e0;
-            int k = rp[r] - e0;
-            T ta = Num<T>::zero(), tb = Num<T>::zero();
-            while (k < e) {
-                ta = Num<T>::add(ta, prod[k]);
-                ++k;
-                if (k < e) { tb = Num<T>::add(tb, prod[k]); ++k; }
-            }
-            yv = Num<T>::add(ta, tb);
+#pragma unroll 4
+            for (int k = rp[r] - e0 + par; k < e; k += 2) acc = Num<T>::add(acc, prod[k]);
+        }
+        const T other = shfl_xor1(acc);
+        if (r < r1 && par == 0) {
+            yv = Num<T>::add(acc, other);
             y[r] = yv;
         }
-    } else if (r < r1) {   // one long row per block
-        const int e = rp[r + 1];
-        int k = rp[r];
+    } else if (threadIdx.x == 0 && r0 < r1) {   // one long row per block
+        const int e = rp[r0 + 1];
+        int k = rp[r0];
         T ta = Num<T>::zero(), tb = Num<T>::zero();
         while (k < e) {
             ta = Num<T>::add(ta, Num<T>::mul(v[k], x[ci[k]]));
@@ -100,12 +131,17 @@ __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const
             if (k < e) { tb = Num<T>::add(tb, Num<T>::mul(v[k], x[ci[k]])); ++k; }
         }
         yv = Num<T>::add(ta, tb);
-        y[r] = yv;
+        y[r0] = yv;
     }
     if (dotv) {
+        // row j's term at thread slot j, then the block tree
+        __syncthreads();
+        T* term = prod;
+        if (r < r1 && par == 0) term[j] = Num<T>::mul(dotv[r], yv);
+        __syncthreads();
         constexpr int K = Parts<T>::K;
         double acc[K] = {};
-        if (r < r1) Parts<T>::put(acc, Num<T>::mul(dotv[r], yv));
+        if ((int)threadIdx.x < r1 - r0) Parts<T>::put(acc, term[threadIdx.x]);
         block_sum<K>(acc);
         if (threadIdx.x == 0)
             for (int k = 0; k < K; ++k) partial[b * K + k] = acc[k];

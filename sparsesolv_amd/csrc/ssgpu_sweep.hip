@@ -24,6 +24,7 @@
 // pass, not bytes: a pass reloads only the inputs a lane still misses, and the
 // register bucket (row length rounded up) keeps the pass short.
 #include <algorithm>
+#include <type_traits>
 
 #include "ssgpu_kernels.h"
 
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
                                                       const int* __restrict__ ecol, const F* __restrict__ eval,
                                                       const F* __restrict__ dval, const T* __restrict__ rhs,
                                                       const T* __restrict__ aux, const T* __restrict__ dd,
-                                                      T* __restrict__ x) {
+                                                      int max_idle, T* __restrict__ x) {
     const int lane = threadIdx.x;
     long spins = 0;
     while (true) {
@@ -135,20 +136,22 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
         RowState<T, F, MODE, KMAX> row;
         row.load(p, active, c, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
         T g[KMAX];
-        unsigned long long miss = active ? (row.len >= 64 ? ~0ull : ((1ull << row.len) - 1ull)) : 0ull;
+        using Mask = typename std::conditional<(KMAX <= 32), unsigned, unsigned long long>::type;
+        constexpr Mask kOne = 1;
+        Mask miss = active ? (row.len >= (int)(8 * sizeof(Mask)) ? ~Mask(0) : ((kOne << row.len) - kOne)) : Mask(0);
         bool done = !active;
         int idle = 0;
         while (true) {
             __asm__ __volatile__("" ::: "memory");
             bool progress = false;
             if (!done && miss) {
-                const unsigned long long want = miss;
+                const Mask want = miss;
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
-                    if (want & (1ull << k)) g[k] = gload(x + row.cols[k]);
+                    if (want & (kOne << k)) g[k] = gload(x + row.cols[k]);
 #pragma unroll
                 for (int k = 0; k < KMAX; ++k)
-                    if ((want & (1ull << k)) && !sent(g[k])) miss &= ~(1ull << k);
+                    if ((want & (kOne << k)) && !sent(g[k])) miss &= ~(kOne << k);
                 progress = miss != want;
             }
             if (!done && miss == 0ull) {
@@ -164,10 +167,14 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
             if (__any(progress)) {
                 idle = 0;
             } else {
-                // back off while nothing arrives: 64, then 128 clocks
-                if (idle < 2) ++idle;
-                if (idle == 1) __builtin_amdgcn_s_sleep(1);
-                else __builtin_amdgcn_s_sleep(2);
+                // back off while nothing arrives (waves far ahead of the level
+                // front go quiet): 64, 128, then up to 64 * 2^(max_idle-2) clocks
+                if (idle < max_idle) ++idle;
+                if (idle <= 1) __builtin_amdgcn_s_sleep(1);
+                else if (idle == 2) __builtin_amdgcn_s_sleep(2);
+                else if (idle == 3) __builtin_amdgcn_s_sleep(4);
+                else if (idle == 4) __builtin_amdgcn_s_sleep(8);
+                else __builtin_amdgcn_s_sleep(16);
             }
         }
     }
@@ -229,12 +236,12 @@ inline int grid_for(int64_t n) {
 // register buckets: the row length rounded up (fewer predicated loads per poll
 // pass shortens the pass, and the pass length is part of every level's cost)
 template <typename T, typename F, int MODE>
-void sweep_kmax(hipStream_t s, int kmax, int g, int n, int nchunks, int* ticket, int* err, const int* srow,
+void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks, int* ticket, int* err, const int* srow,
                 const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval, const T* rhs,
                 const T* aux, const T* dd, T* x) {
 #define SSGPU_SF(K)                                                                                               \
     hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err, srow, \
-                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, max_idle, x)
     if (kmax <= 8) SSGPU_SF(8);
     else if (kmax <= 12) SSGPU_SF(12);
     else if (kmax <= 16) SSGPU_SF(16);
@@ -250,17 +257,17 @@ void sweep_kmax(hipStream_t s, int kmax, int g, int n, int nchunks, int* ticket,
 template <typename T, typename F>
 void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
-                     const T* rhs, const T* aux, const T* dd, T* x) {
+                     const T* rhs, const T* aux, const T* dd, int max_idle, T* x) {
     const int nchunks = (n + kSfRows - 1) / kSfRows;
     const int g = std::max(1, std::min(grid, nchunks));
     if (mode == FWD_DIV_LAST)
-        sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
+        sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
                                        eval, dval, rhs, aux, dd, x);
     else if (mode == BWD_IC)
-        sweep_kmax<T, F, BWD_IC>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol, eval,
+        sweep_kmax<T, F, BWD_IC>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol, eval,
                                  dval, rhs, aux, dd, x);
     else
-        sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
+        sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
                                         eval, dval, rhs, aux, dd, x);
 }
 
@@ -278,16 +285,16 @@ template void launch_sf_prepare<double>(hipStream_t, int, double*, double*, int*
 template void launch_sf_prepare<zc>(hipStream_t, int, zc*, zc*, int*);
 template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
                                          const int*, const int*, const double*, const double*, const double*, const double*,
-                                         const double*, double*);
+                                         const double*, int, double*);
 template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
                                          const int*, const int*, const float*, const float*, const double*, const double*,
-                                         const double*, double*);
+                                         const double*, int, double*);
 template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
                                          const int*, const int*, const zc*, const zc*, const zc*, const zc*,
-                                         const zc*, zc*);
+                                         const zc*, int, zc*);
 template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
                                          const int*, const int*, const fc*, const fc*, const zc*, const zc*,
-                                         const zc*, zc*);
+                                         const zc*, int, zc*);
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);
