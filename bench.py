@@ -9,7 +9,7 @@ performs it (shifted IC(0) factorization + ICCG iterations) with b and x
 resident in HBM.  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--n 95] [--solver iccg|mrtr_ic|mrtr_sgs]
-                  [--precision f64|f32] [--no-cpu]
+                  [--precision f64|f32] [--no-cpu] [--kernels-only]
 """
 from __future__ import annotations
 
@@ -70,6 +70,40 @@ def cpu_baseline(sys_, normB, accel, iters_gpu, solver):
                       f"sweeps as in the reference), extrapolated to {iters_gpu} iterations: {est:.1f} s/solve"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def kernels_only(a, m, s, d_b, prec):
+    """A few launches of the SpMV and both sweeps, then a plain copy of 1 GiB
+    (its FETCH_SIZE / WRITE_SIZE calibrate the counters to bytes)."""
+    import torch
+    d_y = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    m.time_kernel(4, 1, accel=a.accel, precision=prec)   # the factor the sweeps use
+    t = {"spmv": m.time_kernel(0, a.reps, d_b.data_ptr(), d_y.data_ptr()),
+         "sweep_fwd": m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec),
+         "sweep_bwd": m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)}
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(a.reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_ms": t, "copy_bytes_each_way": src.numel() * 8}), flush=True)
+
+
+def traffic_for(n, kernel):
+    """HBM bytes per launch measured by counter runs (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration), if recorded for
+    this grid size."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+        return rec[str(n)][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +117,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=10, help="repetitions for the per-kernel timing")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU layout is design-only",
@@ -112,6 +148,10 @@ def main():
     opts = capi.default_options(accel_factor=a.accel, factor_precision=prec, is_save_log=1)
     log = np.zeros(20002)
     code = SOLVERS[a.solver]
+
+    if a.kernels_only:
+        kernels_only(a, m, s, d_b, prec)
+        return
 
     def step():
         return m.solve_device(code, d_b.data_ptr(), d_x.data_ptr(), conv_cri=a.tol, max_ite=20000, normB=normB,
@@ -186,10 +226,10 @@ def main():
         "levels": {"forward": info["fwd_levels"], "backward": info["bwd_levels"]},
         "roofline": {"bound": "hbm", "kernel": "IC triangular sweep (forward/backward average)",
                      "achieved": gbs_sweep, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_sweep / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
+                     "traffic": traffic_for(a.n, "sweep"), "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
         "roofline_spmv": {"bound": "hbm", "kernel": "CSR SpMV", "achieved": gbs_spmv, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": gbs_spmv / HBM_PEAK_GBS, "algorithmic_bytes": b_spmv,
-                          "avg_ms": t_spmv},
+                          "unit": "GB/s", "frac": gbs_spmv / HBM_PEAK_GBS, "traffic": traffic_for(a.n, "spmv"),
+                          "algorithmic_bytes": b_spmv, "avg_ms": t_spmv},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
