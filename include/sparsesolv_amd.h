@@ -80,6 +80,9 @@ typedef struct {
     int pivot_fixes;             /* pivots with |s| < 1e-12 replaced (SprsPreconditioner.cpp:105-106) */
     double setup_ms;             /* factorization + setup time on the device */
     double iterate_ms;           /* Krylov loop time on the device */
+    double true_residual;        /* ||b - A x|| / normB of the returned x, recomputed in FP64 */
+    int refinements;             /* FP32 factor: outer refinement passes on the FP64 residual */
+    int refine_iterations;       /* Krylov iterations spent in those passes */
 } ssgpu_result;
 
 void ssgpu_options_default(ssgpu_options* o);

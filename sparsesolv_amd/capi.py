@@ -43,6 +43,7 @@ class Result(ctypes.Structure):
         ("status", ctypes.c_int), ("converged", ctypes.c_int), ("iterations", ctypes.c_int),
         ("log_len", ctypes.c_int), ("final_residual", ctypes.c_double), ("accel_used", ctypes.c_double),
         ("pivot_fixes", ctypes.c_int), ("setup_ms", ctypes.c_double), ("iterate_ms", ctypes.c_double),
+        ("true_residual", ctypes.c_double), ("refinements", ctypes.c_int), ("refine_iterations", ctypes.c_int),
     ]
 
 

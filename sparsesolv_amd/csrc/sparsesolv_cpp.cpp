@@ -77,12 +77,12 @@ struct DeviceMatrixCache {
 
 namespace {
 
+// One context per device for the process's lifetime.  They are not destroyed
+// from a static destructor: at process exit the HIP runtime may already be
+// gone, and the driver reclaims the memory anyway.
 struct ContextRegistry {
     std::mutex mu;
     std::map<int, ssgpu_context> ctx;
-    ~ContextRegistry() {
-        for (auto& kv : ctx) ssgpu_context_destroy(kv.second);
-    }
 };
 ContextRegistry& registry() {
     static ContextRegistry r;

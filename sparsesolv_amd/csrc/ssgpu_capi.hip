@@ -55,6 +55,11 @@ struct Context {
     DevBuf d_tickets;           // sync-free sweep tickets (2 ints)
     DevBuf d_err;               // sync-free sweep error flag
     std::mutex mu;
+    // matrices created on this context and not yet destroyed; a context whose
+    // destroy arrives while some are alive is freed with the last of them
+    int live_matrices = 0;
+    bool destroy_pending = false;
+    void* owner = nullptr;   // the C-ABI handle holding this context
 };
 
 
@@ -854,6 +859,8 @@ template <> zc Solver::one<zc>() { return zc{1.0, 0.0}; }
 
 // Full solve with device-resident b, x (front ends MatSolverICCG.cpp:99-168,
 // MatSolverMRTR_ICver.cpp:99-173, MatSolverMRTR_SGSver.cpp:42-69).
+constexpr int kMaxRefine = 3;   // outer refinement passes of an FP32-factor solve
+
 template <typename T>
 int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite, double normB,
                  const T* b, T* x, double* log, int log_cap, ssgpu_result* res) {
@@ -912,6 +919,39 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
         launch_mul_scale<T>(s, n, m.sc.as<T>(), xx);
         launch_copy<T>(s, n, xx, x);
     }
+    // true residual of the returned x, in FP64: ||b - A x|| / normB
+    const double nb = normB > 0.0 ? normB : 1.0;
+    T* rt = S.vec<T>(12);
+    S.spmv<T>(m.val.as<T>(), x, rt);
+    launch_residual<T>(s, n, b, rt, rt);
+    double rn = S.norm<T>(rt);
+    double true_res = rn / nb;
+    // FP32 factor: outer refinement on the FP64 residual (x += A^-1 r through the
+    // same factor), for the solvers whose x is the solution of their recurrence
+    int refinements = 0, refine_its = 0;
+    const bool refinable = o.factor_precision == SSGPU_FACTOR_F32 && !scaled &&
+                           (solver == SSGPU_ICCG || (solver == SSGPU_MRTR_IC && o.mrtr_direction_fix));
+    while (refinable && conv && true_res > conv_cri && refinements < kMaxRefine) {
+        ssgpu_options o2 = o;
+        o2.is_init = 1;
+        o2.is_save_log = 0;
+        o2.is_save_best = 0;
+        o2.diverge_judge_type = 0;
+        o2.conv_normalize_type = 0;
+        Solver R(m, o2);
+        T* d = S.vec<T>(13);
+        const double cc = std::min(0.5 * conv_cri * nb / rn, 0.5);
+        const bool ok = solver == SSGPU_ICCG ? R.iccg_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d)
+                                             : R.mrtr_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d);
+        refine_its += R.iterations;
+        launch_add_inplace<T>(s, n, d, x);
+        ++refinements;
+        S.spmv<T>(m.val.as<T>(), x, rt);
+        launch_residual<T>(s, n, b, rt, rt);
+        rn = S.norm<T>(rt);
+        true_res = rn / nb;
+        if (!ok) break;
+    }
     SSGPU_HIP(hipEventRecord(e2, s));
     SSGPU_HIP(hipEventSynchronize(e2));
     S.check_sweep_error();
@@ -931,6 +971,9 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
         res->pivot_fixes = fixes;
         res->setup_ms = t_setup;
         res->iterate_ms = t_it;
+        res->true_residual = true_res;
+        res->refinements = refinements;
+        res->refine_iterations = refine_its;
     }
     if (log && o.is_save_log)
         for (int i = 0; i < (int)S.log.size() && i < log_cap; ++i) log[i] = S.log[i];
@@ -1010,21 +1053,37 @@ int ssgpu_context_create(int device, ssgpu_context* out) {
     SSGPU_HIP(hipSetDevice(device));
     SSGPU_HIP(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
     SSGPU_HIP(hipHostMalloc((void**)&h->c.h_scal, 64 * sizeof(double), hipHostMallocDefault));
+    h->c.owner = h.get();
     *out = h.release();
     return SSGPU_OK;
     SSGPU_GUARD_END
 }
 
-int ssgpu_context_destroy(ssgpu_context ctx) {
-    if (!ctx) return SSGPU_BAD_ARG;
-    SSGPU_GUARD_BEGIN
+static std::mutex g_lifetime_mu;   // guards live_matrices / destroy_pending of every context
+
+static void context_free(ssgpu_context ctx) {
     (void)hipSetDevice(ctx->c.device);
     (void)hipStreamSynchronize(ctx->c.stream);
     ctx->c.d_scal.release();
     ctx->c.d_count.release();
+    ctx->c.d_tickets.release();
+    ctx->c.d_err.release();
     if (ctx->c.h_scal) (void)hipHostFree(ctx->c.h_scal);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
+}
+
+int ssgpu_context_destroy(ssgpu_context ctx) {
+    if (!ctx) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    {
+        std::lock_guard<std::mutex> lk(g_lifetime_mu);
+        if (ctx->c.live_matrices > 0) {   // freed when its last matrix is destroyed
+            ctx->c.destroy_pending = true;
+            return SSGPU_OK;
+        }
+    }
+    context_free(ctx);
     return SSGPU_OK;
     SSGPU_GUARD_END
 }
@@ -1073,6 +1132,10 @@ int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* 
     if (m.nnz) SSGPU_HIP(hipMemcpyAsync(m.val.p, values, m.vsize * m.nnz, hipMemcpyHostToDevice, ctx->c.stream));
     analyse(m);
     SSGPU_HIP(hipStreamSynchronize(ctx->c.stream));
+    {
+        std::lock_guard<std::mutex> lk(g_lifetime_mu);
+        ++ctx->c.live_matrices;
+    }
     *out = h.release();
     return SSGPU_OK;
     SSGPU_GUARD_END
@@ -1093,9 +1156,16 @@ int ssgpu_matrix_update_values(ssgpu_matrix mh, const void* values) {
 int ssgpu_matrix_destroy(ssgpu_matrix mh) {
     if (!mh) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
-    (void)hipSetDevice(mh->m.ctx->device);
-    (void)hipStreamSynchronize(mh->m.ctx->stream);
+    Context* c = mh->m.ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     delete mh;
+    bool free_ctx = false;
+    {
+        std::lock_guard<std::mutex> lk(g_lifetime_mu);
+        free_ctx = --c->live_matrices == 0 && c->destroy_pending;
+    }
+    if (free_ctx) context_free(static_cast<ssgpu_context>(c->owner));
     return SSGPU_OK;
     SSGPU_GUARD_END
 }

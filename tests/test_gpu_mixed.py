@@ -1,0 +1,76 @@
+"""FP32-factor solves (factor stored and triangular sweeps run in FP32; pivots,
+SpMV, dot products, recurrences and the refinement residual in FP64).
+
+Not a parity mode: the preconditioner differs from the reference's in its
+last bits, so the bar is the stopping criterion itself, checked on the true
+residual ||b - A x|| / ||b|| recomputed in FP64, plus the distance to the
+FP64 solution in the energy-free form ||A (x32 - x64)|| / ||b|| (the curl-curl
+systems are singular, so x itself is only fixed up to gradients).  Each
+result's error against the FP64 oracle is printed.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sparsesolv_amd import capi, voxel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def _sys(kind, n, seed):
+    if kind == "real":
+        return voxel.curlcurl(n, seed=seed, iron_fraction=0.2)
+    return voxel.eddy_current(n, seed=seed)
+
+
+@pytest.mark.parametrize("kind,n", [("real", 10), ("real", 16), ("complex", 10)])
+@pytest.mark.parametrize("solver", [capi.ICCG, capi.MRTR_IC])
+def test_fp32_factor_meets_tolerance(torch_cuda, ctx, kind, n, solver):
+    s = _sys(kind, n, seed=3)
+    print(f"fp32 factor {kind} n={n} N={s.n} solver={solver}", flush=True)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    fix = 1 if solver == capi.MRTR_IC else 0
+    o64 = capi.default_options(accel_factor=1.1, is_save_log=1, mrtr_direction_fix=fix)
+    o32 = capi.default_options(accel_factor=1.1, is_save_log=1, mrtr_direction_fix=fix,
+                               factor_precision=capi.FACTOR_F32)
+    c64, x64, log64, r64 = m.solve(solver, s.rhs, conv_cri=TOL, max_ite=5000, opts=o64)
+    c32, x32, log32, r32 = m.solve(solver, s.rhs, conv_cri=TOL, max_ite=5000, opts=o32)
+    assert c64 and c32
+    A = s.scipy()
+    nb = np.linalg.norm(s.rhs)
+    true32 = np.linalg.norm(s.rhs - A @ x32) / nb
+    # the library's FP64 recompute agrees with an independent one
+    assert abs(r32.true_residual - true32) <= 1e-3 * true32 + 1e-15
+    assert true32 <= TOL, (true32, r32.refinements)
+    # iteration counts stay close to the FP64 path (the preconditioner differs
+    # in its last FP32 bits; complex64 factors move the count by a few percent)
+    assert abs(r32.iterations - r64.iterations) <= max(2, int(0.10 * r64.iterations))
+    err_ax = np.linalg.norm(A @ (x32 - x64)) / nb
+    err_x = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
+    # against the FP64 oracle (reference arithmetic)
+    name = "iccg" if solver == capi.ICCG else "mrtr"
+    _, xo, _ = O.solve(name, s.row_ptr, s.col_idx, s.values, s.rhs, conv_cri=TOL, max_ite=5000,
+                       normB=O.norm(s.rhs), accel_factor=1.1, precondition_type=0, mrtr_direction_fix=fix)
+    err_oracle = np.linalg.norm(A @ (x32 - xo)) / nb
+    print(f"  iters f64 {r64.iterations} f32 {r32.iterations} (+{r32.refine_iterations} in "
+          f"{r32.refinements} refinements); true residual f32 {true32:.2e}; ||A(x32-x64)||/||b|| {err_ax:.2e}; "
+          f"||x32-x64||/||x64|| {err_x:.2e}; vs oracle ||A(x32-xo)||/||b|| {err_oracle:.2e}", flush=True)
+    assert err_ax <= 4 * TOL and err_oracle <= 4 * TOL
+
+
+def test_fp64_true_residual_reported(torch_cuda, ctx):
+    s = voxel.curlcurl(8, seed=1)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    conv, x, _, r = m.solve(capi.ICCG, s.rhs, conv_cri=TOL, opts=capi.default_options(accel_factor=1.2))
+    assert conv and r.refinements == 0
+    true = np.linalg.norm(s.rhs - s.scipy() @ x) / np.linalg.norm(s.rhs)
+    assert abs(r.true_residual - true) <= 1e-3 * true + 1e-15
+
+
+def test_fp32_factor_sgs_is_unsupported(torch_cuda, ctx):
+    s = voxel.curlcurl(6, seed=1)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    with pytest.raises(capi.SolverError):
+        m.solve(capi.MRTR_SGS, s.rhs, opts=capi.default_options(factor_precision=capi.FACTOR_F32))
