@@ -73,6 +73,48 @@ def cpu_baseline(sys_, normB, accel, iters_gpu, solver):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
+def served_run(a, s, code, release):
+    """The same system solved for many right-hand sides through the serving
+    engine: the matrix and its factor stay resident (built once, by the first
+    request), and `slots` solves run at once on separate streams.  Right-hand
+    sides are consistent (A u, u ~ U[-1, 1] seeded).  Measured with the FP32
+    and with the FP64 factor; latency counts from submission (all requests
+    arrive together, so it includes queueing)."""
+    from sparsesolv_amd import capi, serve
+    release()   # free the drop-in's matrix first
+    A = s.scipy()
+    rhs = [A @ np.random.default_rng(1000 + k).uniform(-1.0, 1.0, s.n) for k in range(a.serve_requests)]
+    out = {"slots": a.serve_slots, "requests": a.serve_requests, "solves_per_s": {}, "latency_ms": {},
+           "iterations": {}, "max_true_residual": {}, "setup_s": {}}
+    eng = serve.Engine(slots=a.serve_slots)
+    try:
+        for name, prec in (("f32_factor", capi.FACTOR_F32), ("f64", capi.FACTOR_F64)):
+            def req(b):
+                return serve.Request(key="bench", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                     solver=code, accel=a.accel, precision=prec, conv_cri=a.tol)
+            t0 = time.perf_counter()
+            first = eng.submit(req(rhs[0])).result()   # builds the resident system and factor
+            out["setup_s"][name] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            futs = [eng.submit(req(b)) for b in rhs]
+            resp = [f.result() for f in futs]
+            el = time.perf_counter() - t0
+            bad = [r.error for r in resp if r.error] + ([first.error] if first.error else [])
+            if bad:
+                raise RuntimeError(bad[0])
+            out["solves_per_s"][name] = len(resp) / el
+            out["latency_ms"][name] = {k: 1e3 * v for k, v in serve.percentiles([r.latency_s for r in resp]).items()}
+            out["iterations"][name] = [min(r.result.iterations for r in resp), max(r.result.iterations for r in resp)]
+            out["max_true_residual"][name] = max(r.result.true_residual for r in resp)
+            print(f"# served {name}: {len(resp)} solves in {el:.2f}s ({len(resp) / el:.2f}/s), "
+                  f"p50 {out['latency_ms'][name]['p50']:.0f} ms p99 {out['latency_ms'][name]['p99']:.0f} ms, "
+                  f"iterations {out['iterations'][name]}, max true residual {out['max_true_residual'][name]:.2e}",
+                  file=sys.stderr, flush=True)
+    finally:
+        eng.close()
+    return out
+
+
 def kernels_only(a, m, s, d_b, prec):
     """A few launches of the SpMV and both sweeps, then a plain copy of 1 GiB
     (its FETCH_SIZE / WRITE_SIZE calibrate the counters to bytes)."""
@@ -119,6 +161,8 @@ def main():
     ap.add_argument("--reps", type=int, default=10, help="repetitions for the per-kernel timing")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
+    ap.add_argument("--serve-slots", type=int, default=8, help="concurrent slots of the served run (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU layout is design-only",
@@ -192,6 +236,11 @@ def main():
     print(f"# per-iteration {per_it_ms:.3f} ms; spmv {t_spmv:.3f} ms ({gbs_spmv:.0f} GB/s); "
           f"fwd {t_fwd:.3f} ms bwd {t_bwd:.3f} ms (sweep {gbs_sweep:.0f} GB/s)", file=sys.stderr, flush=True)
 
+    served = None
+    if a.serve_slots > 0 and a.solver != "mrtr_sgs":
+        served = served_run(a, s, code, m.close)
+        m = None
+
     cpu = None
     if not a.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -231,9 +280,13 @@ def main():
                           "unit": "GB/s", "frac": gbs_spmv / HBM_PEAK_GBS, "traffic": traffic_for(a.n, "spmv"),
                           "algorithmic_bytes": b_spmv, "avg_ms": t_spmv},
         "cpu_baseline": cpu,
+        "served": served,
     }
+    if served:
+        served["vs_plain_fp64"] = {k: v / out["value"] for k, v in served["solves_per_s"].items()}
     print(json.dumps(out), flush=True)
-    m.close()
+    if m is not None:
+        m.close()
     ctx.close()
 
 

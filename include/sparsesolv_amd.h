@@ -122,6 +122,28 @@ int ssgpu_solve_device(ssgpu_matrix m, int solver, const ssgpu_options* opt, dou
                        int max_ite, double normB, const void* d_b, void* d_x, double* log,
                        int log_cap, ssgpu_result* res);
 
+/* Cached preconditioner for repeated-matrix solves (serving, batched RHS).
+ * The reference rebuilds the factor on every solve (MatSolverICCG.cpp:109-118,
+ * MatSolverMRTR_ICver.cpp:113-123); these two calls split that: build the
+ * shifted IC(0) factor once (same kernels and result as inside ssgpu_solve;
+ * accel < 0 runs the automatic shift search of MatSolverBase.cpp:344-391),
+ * or for SSGPU_MRTR_SGS the scaled matrix S A S and its triangles
+ * (MatSolverMRTR_SGSver.cpp:42-69), then solve any number of right-hand
+ * sides against it.  ssgpu_solve / ssgpu_solve_device on the same matrix
+ * overwrite the cached preconditioner. */
+int ssgpu_factorize(ssgpu_matrix m, int solver, double accel, int factor_precision, double* accel_used,
+                    int* pivot_fixes);
+
+/* One solve against the preconditioner of ssgpu_factorize, on ctx's stream
+ * and with ctx's work space; b and x are device pointers.  Nothing of the
+ * matrix is written: threads may run solves on one matrix at once through
+ * different contexts (of the matrix's device), and their work overlaps on
+ * the GPU.  Results are the same as ssgpu_solve_device's with that factor
+ * (setup_ms is 0).  Diagonal scaling (is_diag_scale) is not available here. */
+int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix m, int solver, const ssgpu_options* opt,
+                         double conv_cri, int max_ite, double normB, const void* d_b, void* d_x,
+                         double* log, int log_cap, ssgpu_result* res);
+
 /* Kernel-level entry points (device pointers, context stream), used by the
  * parity tests to check each piece against the reference's arithmetic. */
 int ssgpu_spmv_device(ssgpu_matrix m, const void* d_x, void* d_y);

@@ -74,6 +74,8 @@ def lib():
                                         P(ctypes.c_int32)]
         L.ssgpu_solve.argtypes = [V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_solve_device.argtypes = [V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
+        L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
+        L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_spmv_device.argtypes = [V, V, V]
         L.ssgpu_ic_factor_host.argtypes = [V, D, V, V, V, V, P(I)]
         L.ssgpu_ic_apply_device.argtypes = [V, V, V]
@@ -190,6 +192,23 @@ class Matrix:
         cap = 0 if log is None else len(log)
         st = lib().ssgpu_solve_device(self.h, solver, ctypes.byref(o), conv_cri, max_ite, normB, d_b, d_x,
                                       None if log is None else log.ctypes.data, cap, ctypes.byref(res))
+        check(st, allow=(OK, NOT_CONVERGED))
+        return res
+
+    def factorize(self, solver=ICCG, accel=1.05, precision=FACTOR_F64):
+        """Build the cached preconditioner; returns (accel_used, pivot_fixes)."""
+        a, f = ctypes.c_double(), ctypes.c_int()
+        check(lib().ssgpu_factorize(self.h, solver, accel, precision, ctypes.byref(a), ctypes.byref(f)))
+        return a.value, f.value
+
+    def solve_factored(self, ctx: "Context", solver, d_b: int, d_x: int, *, conv_cri=1e-8, max_ite=10000,
+                       normB=0.0, opts=None, log=None):
+        """Solve on ctx's stream against the cached preconditioner (b, x device pointers)."""
+        o = opts or default_options()
+        res = Result()
+        cap = 0 if log is None else len(log)
+        st = lib().ssgpu_solve_factored(ctx.h, self.h, solver, ctypes.byref(o), conv_cri, max_ite, normB, d_b, d_x,
+                                        None if log is None else log.ctypes.data, cap, ctypes.byref(res))
         check(st, allow=(OK, NOT_CONVERGED))
         return res
 

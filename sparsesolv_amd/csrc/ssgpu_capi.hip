@@ -60,6 +60,10 @@ struct Context {
     int live_matrices = 0;
     bool destroy_pending = false;
     void* owner = nullptr;   // the C-ABI handle holding this context
+    // Krylov work vectors and reduction partials of the solves run on this
+    // stream (grown on demand): solves on different contexts never share them
+    std::vector<std::unique_ptr<DevBuf>> work;
+    DevBuf partial;
 };
 
 
@@ -140,8 +144,10 @@ struct Matrix {
     int factor_pivot_fixes = 0;
     double factor_accel = 0.0;
     bool factor_valid = false;
-    std::vector<std::unique_ptr<DevBuf>> work;
-    DevBuf partial;
+    // what the sweep values currently hold: 0 nothing, 1 an IC factor
+    // (prep_precision: FP64 only, or FP64 + FP32 copies), 2 SGS values of S A S
+    int prep_kind = 0;
+    int prep_precision = 0;
 };
 
 // levels of the DAG "row i depends on the entries of its row with index on
@@ -384,24 +390,26 @@ struct Solver {
     std::vector<double> log;
     int iterations = 0;
     double final_res = 0.0;
-    Solver(Matrix& mm, const ssgpu_options& oo) : m(mm), c(mm.ctx), s(mm.ctx->stream), o(oo) {}
+    // c: the context whose stream and work space the solve uses
+    Solver(Matrix& mm, Context* cc, const ssgpu_options& oo) : m(mm), c(cc), s(cc->stream), o(oo) {}
+    Solver(Matrix& mm, const ssgpu_options& oo) : Solver(mm, mm.ctx, oo) {}
 
     template <typename T> T* vec(size_t idx) {
-        while (m.work.size() <= idx) m.work.emplace_back(new DevBuf());
-        DevBuf& b = *m.work[idx];
+        while (c->work.size() <= idx) c->work.emplace_back(new DevBuf());
+        DevBuf& b = *c->work[idx];
         const size_t need = sizeof(T) * (size_t)std::max(m.n, 1);
         if (b.bytes < need) b.alloc(c, need);
         return b.as<T>();
     }
     double* partial() {
         const size_t need = sizeof(double) * 8 * (size_t)std::max(m.nblk, kMaxReduceBlocks);
-        if (m.partial.bytes < need) m.partial.alloc(c, need);
-        return m.partial.as<double>();
+        if (c->partial.bytes < need) c->partial.alloc(c, need);
+        return c->partial.as<double>();
     }
     // finalize K doubles from nb partials and read them back (synchronizes)
     const double* fetch(int nb, int K) {
         if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
-        launch_finalize(s, nb, K, m.partial.as<double>(), c->d_scal.as<double>());
+        launch_finalize(s, nb, K, c->partial.as<double>(), c->d_scal.as<double>());
         SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
         SSGPU_HIP(hipStreamSynchronize(s));
         return c->h_scal;
@@ -546,6 +554,7 @@ struct Solver {
         if (m.sfL.kmax > 0) sf_values<T>(m.sfL, av, false, false);
         if (m.sfUs.kmax > 0) sf_values<T>(m.sfUs, av, false, false);
         m.factor_valid = false;
+        m.prep_kind = 2;
     }
     // sync-free sweep error flag: nonzero if a sweep hit its spin limit (reads back, synchronizes)
     void check_sweep_error() {
@@ -630,6 +639,8 @@ struct Solver {
         m.factor_pivot_fixes = fixes;
         m.factor_accel = accel;
         m.factor_valid = true;
+        m.prep_kind = 1;
+        m.prep_precision = o.factor_precision;
         return fixes;
     }
     template <typename T> bool diag_positive() {
@@ -861,6 +872,49 @@ template <> zc Solver::one<zc>() { return zc{1.0, 0.0}; }
 // MatSolverMRTR_ICver.cpp:99-173, MatSolverMRTR_SGSver.cpp:42-69).
 constexpr int kMaxRefine = 3;   // outer refinement passes of an FP32-factor solve
 
+// True residual ||b - A x|| / normB of the returned x, recomputed in FP64.
+// With an FP32 factor (and no diagonal scaling), outer refinement passes
+// x += A^-1 r through the same factor until it meets conv_cri (at most
+// kMaxRefine), for the solvers whose x is the solution of their recurrence.
+template <typename T>
+double finish_residual(Solver& S, int solver, const ssgpu_options& o, bool unscaled, bool conv, double conv_cri,
+                       int max_ite, double normB, const T* b, T* x, int& refinements, int& refine_its) {
+    Matrix& m = S.m;
+    hipStream_t s = S.s;
+    const int n = m.n;
+    const double nb = normB > 0.0 ? normB : 1.0;
+    T* rt = S.vec<T>(12);
+    S.spmv<T>(m.val.as<T>(), x, rt);
+    launch_residual<T>(s, n, b, rt, rt);
+    double rn = S.norm<T>(rt);
+    double true_res = rn / nb;
+    refinements = refine_its = 0;
+    const bool refinable = o.factor_precision == SSGPU_FACTOR_F32 && unscaled &&
+                           (solver == SSGPU_ICCG || (solver == SSGPU_MRTR_IC && o.mrtr_direction_fix));
+    while (refinable && conv && true_res > conv_cri && refinements < kMaxRefine) {
+        ssgpu_options o2 = o;
+        o2.is_init = 1;
+        o2.is_save_log = 0;
+        o2.is_save_best = 0;
+        o2.diverge_judge_type = 0;
+        o2.conv_normalize_type = 0;
+        Solver R(m, S.c, o2);
+        T* d = S.vec<T>(13);
+        const double cc = std::min(0.5 * conv_cri * nb / rn, 0.5);
+        const bool ok = solver == SSGPU_ICCG ? R.iccg_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d)
+                                             : R.mrtr_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d);
+        refine_its += R.iterations;
+        launch_add_inplace<T>(s, n, d, x);
+        ++refinements;
+        S.spmv<T>(m.val.as<T>(), x, rt);
+        launch_residual<T>(s, n, b, rt, rt);
+        rn = S.norm<T>(rt);
+        true_res = rn / nb;
+        if (!ok) break;
+    }
+    return true_res;
+}
+
 template <typename T>
 int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite, double normB,
                  const T* b, T* x, double* log, int log_cap, ssgpu_result* res) {
@@ -919,39 +973,9 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
         launch_mul_scale<T>(s, n, m.sc.as<T>(), xx);
         launch_copy<T>(s, n, xx, x);
     }
-    // true residual of the returned x, in FP64: ||b - A x|| / normB
-    const double nb = normB > 0.0 ? normB : 1.0;
-    T* rt = S.vec<T>(12);
-    S.spmv<T>(m.val.as<T>(), x, rt);
-    launch_residual<T>(s, n, b, rt, rt);
-    double rn = S.norm<T>(rt);
-    double true_res = rn / nb;
-    // FP32 factor: outer refinement on the FP64 residual (x += A^-1 r through the
-    // same factor), for the solvers whose x is the solution of their recurrence
     int refinements = 0, refine_its = 0;
-    const bool refinable = o.factor_precision == SSGPU_FACTOR_F32 && !scaled &&
-                           (solver == SSGPU_ICCG || (solver == SSGPU_MRTR_IC && o.mrtr_direction_fix));
-    while (refinable && conv && true_res > conv_cri && refinements < kMaxRefine) {
-        ssgpu_options o2 = o;
-        o2.is_init = 1;
-        o2.is_save_log = 0;
-        o2.is_save_best = 0;
-        o2.diverge_judge_type = 0;
-        o2.conv_normalize_type = 0;
-        Solver R(m, o2);
-        T* d = S.vec<T>(13);
-        const double cc = std::min(0.5 * conv_cri * nb / rn, 0.5);
-        const bool ok = solver == SSGPU_ICCG ? R.iccg_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d)
-                                             : R.mrtr_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d);
-        refine_its += R.iterations;
-        launch_add_inplace<T>(s, n, d, x);
-        ++refinements;
-        S.spmv<T>(m.val.as<T>(), x, rt);
-        launch_residual<T>(s, n, b, rt, rt);
-        rn = S.norm<T>(rt);
-        true_res = rn / nb;
-        if (!ok) break;
-    }
+    const double true_res = finish_residual<T>(S, solver, o, !scaled, conv, conv_cri, max_ite, normB, b, x,
+                                               refinements, refine_its);
     SSGPU_HIP(hipEventRecord(e2, s));
     SSGPU_HIP(hipEventSynchronize(e2));
     S.check_sweep_error();
@@ -978,6 +1002,99 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
     if (log && o.is_save_log)
         for (int i = 0; i < (int)S.log.size() && i < log_cap; ++i) log[i] = S.log[i];
     return conv ? OK : NOT_CONVERGED;
+}
+
+// Solve with the preconditioner built by ssgpu_factorize, on context c's
+// stream and work space.  Nothing of the matrix is written, so solves on
+// different contexts may run at once on one matrix.
+template <typename T>
+int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite,
+                   double normB, const T* b, T* x, double* log, int log_cap, ssgpu_result* res) {
+    Solver S(m, c, o);
+    hipStream_t s = c->stream;
+    const int n = m.n;
+    hipEvent_t e1, e2;
+    SSGPU_HIP(hipEventCreate(&e1));
+    SSGPU_HIP(hipEventCreate(&e2));
+    SSGPU_HIP(hipEventRecord(e1, s));
+    bool conv = false;
+    if (solver == SSGPU_MRTR_SGS) {
+        if (m.prep_kind != 2) throw Error{BAD_ARG, "no SGS preparation: call ssgpu_factorize with SSGPU_MRTR_SGS"};
+        T* bs = S.vec<T>(14);
+        T* xs = S.vec<T>(10);
+        launch_copy<T>(s, n, b, bs);
+        launch_mul_scale<T>(s, n, m.sc.as<T>(), bs);   // b~ = s o b (SprsPreconditioner.cpp:163-184)
+        launch_copy<T>(s, n, x, xs);
+        if (!o.is_init) launch_div_scale<T>(s, n, m.sc.as<T>(), xs);
+        const double nB = S.norm<T>(bs);
+        conv = S.sgs_main<T>(conv_cri, max_ite, nB, m.scaledA.as<T>(), bs, xs);
+        launch_mul_scale<T>(s, n, m.sc.as<T>(), xs);
+        launch_copy<T>(s, n, xs, x);
+    } else {
+        if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
+        if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
+        if (o.is_diag_scale) throw Error{UNSUPPORTED, "diagonal scaling is not available on a cached factor"};
+        conv = solver == SSGPU_ICCG ? S.iccg_main<T>(conv_cri, max_ite, normB, m.val.as<T>(), b, x)
+                                    : S.mrtr_main<T>(conv_cri, max_ite, normB, m.val.as<T>(), b, x);
+    }
+    int refinements = 0, refine_its = 0;
+    const double true_res = finish_residual<T>(S, solver, o, solver != SSGPU_MRTR_SGS, conv, conv_cri, max_ite,
+                                               normB, b, x, refinements, refine_its);
+    SSGPU_HIP(hipEventRecord(e2, s));
+    SSGPU_HIP(hipEventSynchronize(e2));
+    S.check_sweep_error();
+    float t_it = 0.f;
+    (void)hipEventElapsedTime(&t_it, e1, e2);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (res) {
+        res->converged = conv ? 1 : 0;
+        res->status = conv ? OK : NOT_CONVERGED;
+        res->iterations = S.iterations;
+        res->log_len = o.is_save_log ? (int)S.log.size() : 0;
+        res->final_residual = S.final_res;
+        res->accel_used = solver == SSGPU_MRTR_SGS ? 0.0 : m.factor_accel;
+        res->pivot_fixes = solver == SSGPU_MRTR_SGS ? 0 : m.factor_pivot_fixes;
+        res->setup_ms = 0.0;
+        res->iterate_ms = t_it;
+        res->true_residual = true_res;
+        res->refinements = refinements;
+        res->refine_iterations = refine_its;
+    }
+    if (log && o.is_save_log)
+        for (int i = 0; i < (int)S.log.size() && i < log_cap; ++i) log[i] = S.log[i];
+    return conv ? OK : NOT_CONVERGED;
+}
+
+// Build the preconditioner a cached solve uses: the shifted IC(0) factor
+// (automatic shift when accel < 0) or, for SGS-MRTR, S A S and its triangles.
+template <typename T> void factorize(Matrix& m, int solver, double accel, int precision, double* accel_used,
+                                     int* fixes) {
+    ssgpu_options o;
+    ssgpu_options_default(&o);
+    o.factor_precision = precision;
+    Solver S(m, o);
+    hipStream_t s = m.ctx->stream;
+    const int n = m.n;
+    if (solver == SSGPU_MRTR_SGS) {
+        if (!m.lower_diag_ok || !m.upper_diag_ok)
+            throw Error{NO_DIAGONAL, "SGS-MRTR needs every row's diagonal entry"};
+        if (m.sc.bytes < sizeof(T) * n) m.sc.alloc(m.ctx, sizeof(T) * n);
+        if (m.scaledA.bytes < sizeof(T) * m.nnz) m.scaledA.alloc(m.ctx, sizeof(T) * m.nnz);
+        launch_diag_scale<T>(s, n, m.diag.as<int>(), m.val.as<T>(), nullptr, m.sc.as<T>(), nullptr);
+        launch_scale_matrix<T>(s, n, m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), m.sc.as<T>(),
+                               m.scaledA.as<T>());
+        S.sgs_values<T>(m.scaledA.as<T>());
+        if (accel_used) *accel_used = 0.0;
+        if (fixes) *fixes = 0;
+    } else {
+        if (!m.lower_diag_ok) throw Error{NO_DIAGONAL, "there is not diagonal element!"};
+        const double a = accel < 0.0 ? S.auto_accel<T>(m.val.as<T>(), accel) : (S.factor<T>(m.val.as<T>(), accel), accel);
+        if (accel_used) *accel_used = a;
+        if (fixes) *fixes = m.factor_pivot_fixes;
+    }
+    SSGPU_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace ssgpu
@@ -1202,6 +1319,41 @@ int ssgpu_solve_device(ssgpu_matrix mh, int solver, const ssgpu_options* opt, do
         return solve_device<double>(m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b, (double*)d_x, log,
                                     log_cap, res);
     return solve_device<zc>(m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log, log_cap, res);
+    SSGPU_GUARD_END
+}
+
+int ssgpu_factorize(ssgpu_matrix mh, int solver, double accel, int factor_precision, double* accel_used,
+                    int* pivot_fixes) {
+    if (!mh || solver < SSGPU_ICCG || solver > SSGPU_MRTR_SGS) return SSGPU_BAD_ARG;
+    if (factor_precision != SSGPU_FACTOR_F64 && factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
+    if (solver == SSGPU_MRTR_SGS && factor_precision != SSGPU_FACTOR_F64) return SSGPU_UNSUPPORTED;
+    SSGPU_GUARD_BEGIN
+    Matrix& m = mh->m;
+    std::lock_guard<std::mutex> lk(m.ctx->mu);
+    SSGPU_HIP(hipSetDevice(m.ctx->device));
+    if (m.dtype == SSGPU_R64) factorize<double>(m, solver, accel, factor_precision, accel_used, pivot_fixes);
+    else factorize<zc>(m, solver, accel, factor_precision, accel_used, pivot_fixes);
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix mh, int solver, const ssgpu_options* opt, double conv_cri,
+                         int max_ite, double normB, const void* d_b, void* d_x, double* log, int log_cap,
+                         ssgpu_result* res) {
+    if (!ctx || !mh || !d_b || !d_x) return SSGPU_BAD_ARG;
+    const int ck = check_opts(opt, solver);
+    if (ck != SSGPU_OK) return ck;
+    SSGPU_GUARD_BEGIN
+    Matrix& m = mh->m;
+    Context* c = &ctx->c;
+    if (c->device != m.ctx->device) throw Error{BAD_ARG, "context and matrix are on different devices"};
+    std::lock_guard<std::mutex> lk(c->mu);
+    SSGPU_HIP(hipSetDevice(c->device));
+    if (m.dtype == SSGPU_R64)
+        return solve_factored<double>(c, m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b,
+                                      (double*)d_x, log, log_cap, res);
+    return solve_factored<zc>(c, m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log, log_cap,
+                              res);
     SSGPU_GUARD_END
 }
 

@@ -382,7 +382,7 @@ __global__ void k_diag_scale(int n, const int* __restrict__ diag_idx, const T* _
         }
         const T s = from_real<T>(t2);
         sc[i] = s;
-        bs[i] = Num<T>::mul(b[i], s);
+        if (b) bs[i] = Num<T>::mul(b[i], s);
     }
 }
 

@@ -1,0 +1,126 @@
+"""Cached-factor solves and the serving loop.
+
+* ssgpu_factorize + ssgpu_solve_factored return exactly what ssgpu_solve_device
+  returns (same factor, same kernels), for all three solvers.
+* Solves on several contexts at once on one matrix return, bit for bit, what
+  each returns alone (work space is per context; the factor is only read).
+* The serving engine: every response equals the solo solve of its request,
+  in a mix of real / complex systems, repeated matrices (factor cache hits)
+  and an HBM budget small enough to force evictions.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+from sparsesolv_amd import capi, serve, voxel
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(s, seed):
+    """Consistent right-hand side in range(A): A u, u ~ U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1, 1, s.n)
+    if s.is_complex:
+        u = u + 1j * rng.uniform(-1, 1, s.n)
+    return s.scipy() @ u
+
+
+def _solo(torch, ctx, s, b, solver, accel, precision=capi.FACTOR_F64):
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    d_b = torch.from_numpy(b).cuda()
+    d_x = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    o = capi.default_options(accel_factor=accel, factor_precision=precision)
+    r = m.solve_device(solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=5000,
+                       normB=float(np.linalg.norm(b)), opts=o)
+    ctx.synchronize()
+    x = d_x.cpu().numpy()
+    m.close()
+    return x, r
+
+
+@pytest.mark.parametrize("solver", [capi.ICCG, capi.MRTR_IC, capi.MRTR_SGS])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_factored_equals_solve_device(torch_cuda, ctx, solver, cplx):
+    s = voxel.eddy_current(7, seed=4) if cplx else voxel.curlcurl(8, seed=4)
+    b = _rhs(s, 1)
+    print(f"factored solver={solver} cplx={cplx} N={s.n}", flush=True)
+    x_ref, r_ref = _solo(torch_cuda, ctx, s, b, solver, 1.1)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    m.factorize(solver, 1.1)
+    c2 = capi.Context(0)
+    d_b = torch_cuda.from_numpy(b).cuda()
+    d_x = torch_cuda.zeros_like(d_b)
+    torch_cuda.cuda.synchronize()
+    r = m.solve_factored(c2, solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=5000,
+                         normB=float(np.linalg.norm(b)), opts=capi.default_options(accel_factor=1.1))
+    x = d_x.cpu().numpy()
+    assert r.converged == r_ref.converged and r.iterations == r_ref.iterations
+    assert np.array_equal(x, x_ref)
+    assert r.true_residual == r_ref.true_residual
+    m.close()
+    c2.close()
+
+
+def test_concurrent_contexts_match_solo(torch_cuda, ctx):
+    s = voxel.curlcurl(12, seed=5)
+    rhs = [_rhs(s, k) for k in range(6)]
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    m.factorize(capi.ICCG, 1.2)
+    ctxs = [capi.Context(0) for _ in range(3)]
+    o = capi.default_options(accel_factor=1.2)
+
+    def run(c, b):
+        d_b = torch_cuda.from_numpy(b).cuda()
+        d_x = torch_cuda.zeros_like(d_b)
+        torch_cuda.cuda.synchronize()
+        m.solve_factored(c, capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), normB=float(np.linalg.norm(b)), opts=o)
+        return d_x.cpu().numpy()
+
+    solo = [run(ctxs[0], b) for b in rhs]
+    out = [None] * len(rhs)
+
+    def worker(i):
+        out[i] = run(ctxs[i % 3], rhs[i])
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(len(rhs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for a, b in zip(out, solo):
+        assert np.array_equal(a, b)
+    for c in ctxs:
+        c.close()
+    m.close()
+
+
+def test_serving_mix_matches_solo(torch_cuda, ctx):
+    systems = {"r8": voxel.curlcurl(8, seed=1), "r11": voxel.curlcurl(11, seed=2),
+               "c7": voxel.eddy_current(7, seed=3)}
+    plan = []
+    for k in range(14):
+        key = ["r8", "r11", "c7"][k % 3]
+        solver = [capi.ICCG, capi.MRTR_IC, capi.MRTR_SGS][(k // 3) % 3]
+        prec = capi.FACTOR_F32 if (k % 4 == 1 and solver != capi.MRTR_SGS) else capi.FACTOR_F64
+        plan.append((key, solver, prec, _rhs(systems[key], 100 + k)))
+    # a budget that holds about two systems: the mix evicts and rebuilds
+    biggest = max(serve._system_bytes(s.nnz, s.n, s.is_complex) for s in systems.values())
+    eng = serve.Engine(slots=4, hbm_budget=int(2.2 * biggest))
+    futs = []
+    for key, solver, prec, b in plan:
+        s = systems[key]
+        futs.append(eng.submit(serve.Request(key=key, row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                             solver=solver, accel=1.1, precision=prec)))
+    resp = [f.result(timeout=120) for f in futs]
+    eng.close()
+    assert eng.peak_resident <= eng.budget
+    hits = sum(r.cache_hit for r in resp)
+    print(f"serving mix: {len(resp)} requests, {hits} cache hits, peak resident {eng.peak_resident} B", flush=True)
+    for (key, solver, prec, b), r in zip(plan, resp):
+        assert r.error is None, r.error
+        x_solo, r_solo = _solo(torch_cuda, ctx, systems[key], b, solver, 1.1, prec)
+        assert r.result.iterations == r_solo.iterations
+        assert np.array_equal(r.x, x_solo), (key, solver, prec)
