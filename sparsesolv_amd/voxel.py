@@ -86,11 +86,13 @@ def _random_box(rng, n, size):
 
 
 def make_params(n, *, h=1.0, ordering=0, seed=0, iron_block=None, iron_fraction=0.0,
-                conductor_block=None, sigma=0.0, omega=0.0, coil=True):
+                conductor_block=None, sigma=0.0, omega=0.0, coil=True, coil_seed=None):
     """Seeded layout.  iron_block: edge length of one iron cube at a random
     position; iron_fraction: fill about that volume fraction with random iron
     cubes of edge max(2, n//8).  conductor_block: edge length of one
-    conductor cube at a random position (eddy-current form)."""
+    conductor cube at a random position (eddy-current form).  coil_seed:
+    place the coil from its own seed (slab height and a centre offset) so
+    that right-hand sides vary while the matrix stays the same."""
     rng = np.random.default_rng(seed)
     iron = []
     if iron_block:
@@ -122,7 +124,13 @@ def make_params(n, *, h=1.0, ordering=0, seed=0, iron_block=None, iron_fraction=
     L = n * h
     thick = max(1, n // 10)
     z0 = int(rng.integers(1, max(2, n - thick - 1)))
-    coil_p = dict(cx=0.5 * L, cy=0.5 * L, r_in=0.18 * L, r_out=0.3 * L,
+    cx = cy = 0.5 * L
+    if coil_seed is not None:
+        crng = np.random.default_rng(coil_seed)
+        z0 = int(crng.integers(1, max(2, n - thick - 1)))
+        cx += float(crng.uniform(-0.15, 0.15)) * L
+        cy += float(crng.uniform(-0.15, 0.15)) * L
+    coil_p = dict(cx=cx, cy=cy, r_in=0.18 * L, r_out=0.3 * L,
                   zc0=z0 * h, zc1=(z0 + thick) * h, ampere_turns=1000.0 if coil else 0.0)
     return dict(n=n, h=h, ordering=ordering, iron=iron, cond=cond, sigma=sigma, omega=omega,
                 nu0=NU0, nu_iron=NU0 / 1000.0, **coil_p)
@@ -161,12 +169,12 @@ def build(params: dict, complex_values: bool = False) -> VoxelSystem:
     return VoxelSystem(N, rp, ci, vals, rhs, params["n"], dict(params))
 
 
-def curlcurl(n, *, seed=0, ordering=0, h=1.0, iron_block=None, iron_fraction=0.0):
+def curlcurl(n, *, seed=0, ordering=0, h=1.0, iron_block=None, iron_fraction=0.0, coil_seed=None):
     """Real magnetostatic curl-curl system (iron blocks, loop coil)."""
     if iron_block is None and iron_fraction == 0.0:
         iron_block = max(1, round(0.3 * n))
     return build(make_params(n, h=h, ordering=ordering, seed=seed, iron_block=iron_block,
-                             iron_fraction=iron_fraction))
+                             iron_fraction=iron_fraction, coil_seed=coil_seed))
 
 
 def eddy_current(n, *, seed=0, ordering=0, h=1.0e-3, sigma=5.8e7, freq=50.0, block=None):

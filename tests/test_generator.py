@@ -161,3 +161,11 @@ def test_coil_current_is_divergence_free_loop():
     s = voxel.curlcurl(12, seed=5)
     assert np.count_nonzero(np.abs(s.rhs) > 1e-12) > 0
     assert math.isfinite(float(np.linalg.norm(s.rhs)))
+
+
+def test_coil_seed_changes_only_the_rhs():
+    a = voxel.curlcurl(9, seed=4, iron_fraction=0.2)
+    b = voxel.curlcurl(9, seed=4, iron_fraction=0.2, coil_seed=11)
+    assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col_idx, b.col_idx)
+    assert np.array_equal(a.values, b.values)
+    assert np.linalg.norm(a.rhs - b.rhs) > 0.1 * np.linalg.norm(a.rhs)
