@@ -144,6 +144,19 @@ int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix m, int solver, const ss
                          double conv_cri, int max_ite, double normB, const void* d_b, void* d_x,
                          double* log, int log_cap, ssgpu_result* res);
 
+/* Block ICCG / COCG: nrhs (1..32) right-hand sides against the factor of
+ * ssgpu_factorize in one pass of kernels per iteration (the sweeps carry
+ * every column through the dependency chain at once).  d_B and d_X hold
+ * nrhs vectors of n entries one after the other (entry i of vector j at
+ * j * n + i); normB[j] as for
+ * ssgpu_solve; res[j] per column (no residual log).  Column j returns the
+ * same x, iteration count and residual as ssgpu_solve_factored on that
+ * column alone.  No counterpart in the reference, which solves one
+ * right-hand side per call (MatSolverBase.hpp:142-150). */
+int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* opt, double conv_cri,
+                      int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X,
+                      ssgpu_result* res);
+
 /* Kernel-level entry points (device pointers, context stream), used by the
  * parity tests to check each piece against the reference's arithmetic. */
 int ssgpu_spmv_device(ssgpu_matrix m, const void* d_x, void* d_y);

@@ -37,7 +37,7 @@ def _run(cmd):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_capi.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_capi.hip")]
     out = os.path.join(LIB, "libsparsesolv_amd.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])

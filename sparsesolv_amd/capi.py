@@ -76,6 +76,7 @@ def lib():
         L.ssgpu_solve_device.argtypes = [V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
         L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
+        L.ssgpu_solve_block.argtypes = [V, V, P(Options), D, I, I, V, V, V, P(Result)]
         L.ssgpu_spmv_device.argtypes = [V, V, V]
         L.ssgpu_ic_factor_host.argtypes = [V, D, V, V, V, V, P(I)]
         L.ssgpu_ic_apply_device.argtypes = [V, V, V]
@@ -211,6 +212,18 @@ class Matrix:
                                         None if log is None else log.ctypes.data, cap, ctypes.byref(res))
         check(st, allow=(OK, NOT_CONVERGED))
         return res
+
+    def solve_block(self, ctx: "Context", d_B: int, d_X: int, normB, *, conv_cri=1e-8, max_ite=10000, opts=None):
+        """Block ICCG/COCG over k = len(normB) right-hand sides stored one after the other in d_B (a k x n
+        C-ordered device array); returns one Result per right-hand side."""
+        o = opts or default_options()
+        k = len(normB)
+        nb = np.ascontiguousarray(normB, dtype=np.float64)
+        res = (Result * k)()
+        st = lib().ssgpu_solve_block(ctx.h, self.h, ctypes.byref(o), conv_cri, max_ite, k, nb.ctypes.data, d_B, d_X,
+                                     res)
+        check(st, allow=(OK, NOT_CONVERGED))
+        return list(res)
 
     def spmv_device(self, d_x: int, d_y: int):
         check(lib().ssgpu_spmv_device(self.h, d_x, d_y))

@@ -64,6 +64,9 @@ struct Context {
     // stream (grown on demand): solves on different contexts never share them
     std::vector<std::unique_ptr<DevBuf>> work;
     DevBuf partial;
+    // block solves: per-column scalars (device + pinned staging), ticket counters
+    DevBuf d_bscal, d_btickets, d_bpartial;
+    void* h_bscal = nullptr;
 };
 
 
@@ -138,6 +141,8 @@ struct Matrix {
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
     int sweep_grid = 128;     // persistent one-wave workgroups per sweep
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
+    int block_kc = 0;         // block sweeps: columns per lane (0: 1)
+    int block_grid = 0;       // block sweeps: workgroups per column group (0: about 512 waves in all)
     bool uic_built = false, us_built = false;
     // factor / preconditioner values and work space (grown on demand)
     DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;
@@ -275,6 +280,8 @@ static void analyse(Matrix& m) {
     upload(c, m.blk, blk);
     if (const char* g = std::getenv("SSGPU_SWEEP_GRID")) m.sweep_grid = std::max(1, std::atoi(g));
     if (const char* g = std::getenv("SSGPU_SWEEP_IDLE")) m.sweep_idle = std::min(5, std::max(1, std::atoi(g)));
+    if (const char* g = std::getenv("SSGPU_BLOCK_KC")) m.block_kc = std::atoi(g);
+    if (const char* g = std::getenv("SSGPU_BLOCK_GRID")) m.block_grid = std::max(1, std::atoi(g));
     std::vector<int> diag(n, -1);
     for (int i = 0; i < n; ++i)
         for (int k = rp[i]; k < rp[i + 1]; ++k)
@@ -1067,6 +1074,278 @@ int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, do
     return conv ? OK : NOT_CONVERGED;
 }
 
+// Block ICCG / COCG: nrhs right-hand sides against the cached IC factor, in
+// kb >= nrhs padded columns (each stored whole: column c at offset c * n).  Column j runs exactly the solve
+// iccg_main runs on it alone (same kernels per column, same reduction trees,
+// same host recurrences and bookkeeping); a column that converges or
+// diverges stops updating its x while the others go on.
+template <typename T> struct BlockIccg {
+    Solver& S;
+    Context* c;
+    Matrix& m;
+    hipStream_t s;
+    int n, kb, P;
+    int kc;   // columns per lane of the block sweeps
+    using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
+    explicit BlockIccg(Solver& SS, int kbb) : S(SS), c(SS.c), m(SS.m), s(SS.s), n(SS.m.n), kb(kbb), P(kparts<T>()) {
+        // one column per lane measured best (shortest polling pass; the
+        // columns' dependency chains run side by side in separate groups)
+        const int lim = sizeof(T) == 8 ? 4 : 2;
+        kc = m.block_kc > 0 ? m.block_kc : 1;
+        if (kc > lim || (kc != 1 && kc != 2 && kc != 4)) kc = 1;
+        while (kc > 1 && kb % kc) kc /= 2;
+    }
+
+    T* bvec(size_t idx) {   // n x kb work array from the context's pool
+        while (c->work.size() <= idx) c->work.emplace_back(new DevBuf());
+        DevBuf& b = *c->work[idx];
+        const size_t need = sizeof(T) * (size_t)std::max(n, 1) * kb;
+        if (b.bytes < need) b.alloc(c, need);
+        return b.as<T>();
+    }
+    double* bpartial() {
+        const size_t need = sizeof(double) * 64 * (size_t)std::max(m.nblk, kMaxReduceBlocks);
+        if (c->d_bpartial.bytes < need) c->d_bpartial.alloc(c, need);
+        return c->d_bpartial.as<double>();
+    }
+    int* btickets() {
+        if (c->d_btickets.bytes == 0) c->d_btickets.alloc(c, 64 * sizeof(int));
+        return c->d_btickets.as<int>();
+    }
+    void ensure_scal() {
+        if (c->d_bscal.bytes == 0) c->d_bscal.alloc(c, 4096);
+        if (!c->h_bscal) SSGPU_HIP(hipHostMalloc(&c->h_bscal, 4096, hipHostMallocDefault));
+        if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
+    }
+    // finalize kb*P partial sums over nb blocks and read them back (synchronizes)
+    std::vector<double> fetch(int nb) {
+        const int K = kb * P;
+        launch_finalize(s, nb, K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
+        std::vector<double> out(K);
+        SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+        SSGPU_HIP(hipStreamSynchronize(s));
+        std::copy(c->h_scal, c->h_scal + K, out.begin());
+        return out;
+    }
+    std::vector<T> dots(const std::vector<double>& v) {
+        std::vector<T> out(kb);
+        for (int j = 0; j < kb; ++j) out[j] = Host<T>::from(&v[(size_t)j * P]);
+        return out;
+    }
+    // per-column scalars and active flags to the device (one copy)
+    void put_scalars(const std::vector<T>& a, const std::vector<int>& act) {
+        char* h = static_cast<char*>(c->h_bscal);
+        std::memcpy(h, a.data(), sizeof(T) * kb);
+        std::memcpy(h + 1024, act.data(), sizeof(int) * kb);
+        SSGPU_HIP(hipMemcpyAsync(c->d_bscal.p, h, 1024 + sizeof(int) * kb, hipMemcpyHostToDevice, s));
+    }
+    const T* d_scalars() { return c->d_bscal.as<T>(); }
+    const int* d_act() { return reinterpret_cast<const int*>(static_cast<char*>(c->d_bscal.p) + 1024); }
+
+    void ic_apply(const T* r, T* z, T* tmp) {
+        launch_sf_prepare<T>(s, n * kb, tmp, z, S.tickets());
+        int* tk = btickets();
+        launch_fill<int>(s, 64, tk, 0);
+        const int ng = kb / kc;
+        // about 512 polling waves in all (fewer per group as groups multiply)
+        const int grid = m.block_grid > 0 ? m.block_grid : std::min(m.sweep_grid, std::max(32, 512 / ng));
+        const bool f32 = S.o.factor_precision == SSGPU_FACTOR_F32;
+        if (f32) {
+            launch_sweep_block<T, F>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), m.sfL.srow,
+                                     m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+                                     m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r, nullptr, nullptr, tmp);
+            launch_sweep_block<T, F>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(),
+                                     m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
+                                     m.sfUic.ecol.as<int>(), m.sfUic.eval32.as<F>(), nullptr, nullptr, tmp,
+                                     m.diagD.as<T>(), z);
+        } else {
+            launch_sweep_block<T, T>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), m.sfL.srow,
+                                     m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+                                     m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr, nullptr, tmp);
+            launch_sweep_block<T, T>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(),
+                                     m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
+                                     m.sfUic.ecol.as<int>(), m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
+                                     m.diagD.as<T>(), z);
+        }
+    }
+
+    struct Col {
+        bool active = true, conv = false;
+        int iterations = 0;
+        double normalizer = 1.0, abs_conv = 0.0, best = 1.0e+6, final_res = 0.0;
+        int bad = 0;
+    };
+
+    // B, X: kb columns of n entries (padding columns of B are zero)
+    std::vector<Col> run(double conv_cri, int max_ite, const std::vector<double>& normB, const T* B, T* X) {
+        const ssgpu_options& o = S.o;
+        ensure_scal();
+        T *P_ = bvec(20), *R = bvec(21), *Z = bvec(22), *AP = bvec(23), *TMP = bvec(24);
+        T* BEST = o.is_save_best ? bvec(25) : nullptr;
+        const int64_t nk = (int64_t)n * kb;
+        std::vector<Col> col(kb);
+        if (o.is_init) launch_fill<T>(s, nk, X, Host<T>::zero());
+        if (BEST) launch_fill<T>(s, nk, BEST, Host<T>::zero());
+        if (!o.is_init) {
+            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), X,
+                                AP, nullptr, nullptr);
+            launch_residual<T>(s, (int)nk, B, AP, R);
+        } else {
+            launch_residual<T>(s, (int)nk, B, nullptr, R);
+        }
+        bpartial();
+        launch_norm2_cols<T>(s, kb, n, n, R, c->d_bpartial.as<double>());
+        {
+            const int K = kb;   // norms: one part per column
+            launch_finalize(s, reduce_blocks(n), K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
+            SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+            SSGPU_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<double> pure(c->h_scal, c->h_scal + kb);
+        constexpr double kSmallAbs = 1.0e-20;
+        for (int j = 0; j < kb; ++j) {   // the prologue of Solver::prologue, per column
+            Col& q = col[j];
+            pure[j] = std::sqrt(pure[j]);
+            const double nB = normB[j];
+            if (nB <= 1.0e-30 && pure[j] <= kSmallAbs) { q.active = false; q.conv = true; continue; }
+            q.abs_conv = (nB * conv_cri * 0.9 < kSmallAbs ? kSmallAbs : nB * conv_cri * 0.9);
+            if (pure[j] / nB < conv_cri * 0.5 || pure[j] < q.abs_conv * 0.5) { q.active = false; q.conv = true; continue; }
+            if (o.conv_normalize_type == 1) q.normalizer = pure[j];
+            else if (o.conv_normalize_type == 2) q.normalizer = o.conv_normalize_const;
+            else q.normalizer = nB;
+        }
+        auto any_active = [&]() {
+            for (auto& q : col) if (q.active) return true;
+            return false;
+        };
+        if (!any_active()) return col;
+        ic_apply(R, P_, TMP);
+        launch_copy<T>(s, nk, P_, Z);
+        launch_dot_cols<T>(s, kb, n, n, R, Z, c->d_bpartial.as<double>());
+        std::vector<T> rur0 = dots(fetch(reduce_blocks(n)));
+        std::vector<T> sc(kb, Host<T>::zero());
+        std::vector<int> act(kb, 0);
+        for (int it = 0; it < max_ite && any_active(); ++it) {
+            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(),
+                                P_, AP, P_, c->d_bpartial.as<double>());
+            const std::vector<T> t2 = dots(fetch(m.nblk));
+            for (int j = 0; j < kb; ++j) {
+                act[j] = col[j].active ? 1 : 0;
+                sc[j] = col[j].active ? Host<T>::div(rur0[j], t2[j]) : Host<T>::zero();
+            }
+            put_scalars(sc, act);
+            launch_cg_update_cols<T>(s, kb, n, n, d_scalars(), d_act(), P_, AP, X, R, c->d_bpartial.as<double>());
+            std::vector<double> pn;
+            {
+                launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
+                SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * kb, hipMemcpyDeviceToHost, s));
+                SSGPU_HIP(hipStreamSynchronize(s));
+                pn.assign(c->h_scal, c->h_scal + kb);
+            }
+            for (int j = 0; j < kb; ++j) {   // Solver::monitor, per column
+                Col& q = col[j];
+                if (!q.active) continue;
+                q.iterations = it + 1;
+                const double p = std::sqrt(pn[j]);
+                const double normR = p / q.normalizer;
+                q.final_res = normR;
+                if (normR < conv_cri || p < q.abs_conv) { q.active = false; q.conv = true; continue; }
+                if (normR < q.best) {
+                    q.best = normR;
+                    if (BEST) launch_copy<T>(s, n, X + (size_t)j * n, BEST + (size_t)j * n);
+                }
+                if (o.diverge_judge_type == 1) {
+                    if (normR < q.best * o.bad_div_val) q.bad = 0;
+                    if (normR >= q.best * o.bad_div_val) q.bad++;
+                    if (q.bad >= o.bad_div_count_thres) { q.active = false; continue; }
+                }
+            }
+            if (!any_active()) break;
+            ic_apply(R, Z, TMP);
+            launch_dot_cols<T>(s, kb, n, n, R, Z, c->d_bpartial.as<double>());
+            const std::vector<T> rur1 = dots(fetch(reduce_blocks(n)));
+            for (int j = 0; j < kb; ++j) {
+                act[j] = col[j].active ? 1 : 0;
+                sc[j] = col[j].active ? Host<T>::div(rur1[j], rur0[j]) : Host<T>::zero();
+                if (col[j].active) rur0[j] = rur1[j];
+            }
+            put_scalars(sc, act);
+            launch_cg_pupdate_cols<T>(s, kb, n, n, d_scalars(), d_act(), Z, P_);
+        }
+        if (BEST)
+            for (int j = 0; j < kb; ++j)
+                if (!col[j].conv) launch_copy<T>(s, n, BEST + (size_t)j * n, X + (size_t)j * n);
+        return col;
+    }
+};
+
+// Block solve entry: copy to kb padded columns, run, copy back, true residuals.
+template <typename T>
+int solve_block(Context* c, Matrix& m, const ssgpu_options& o, double conv_cri, int max_ite, int nrhs,
+                const double* normB, const T* b, T* x, ssgpu_result* res) {
+    if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
+    if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+        throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
+    if (o.is_diag_scale) throw Error{UNSUPPORTED, "diagonal scaling is not available on a cached factor"};
+    if (!m.sfL.kmax || !m.sfUic.kmax || m.sfL.kmax > 32 || m.sfUic.kmax > 32)
+        throw Error{UNSUPPORTED, "block solves need rows of at most 32 off-diagonal entries"};
+    int kb = 1;
+    while (kb < nrhs) kb *= 2;
+    if (kb > 32) throw Error{BAD_ARG, "at most 32 right-hand sides per block solve"};
+    Solver S(m, c, o);
+    BlockIccg<T> B(S, kb);
+    hipStream_t s = c->stream;
+    const int n = m.n;
+    hipEvent_t e1, e2;
+    SSGPU_HIP(hipEventCreate(&e1));
+    SSGPU_HIP(hipEventCreate(&e2));
+    SSGPU_HIP(hipEventRecord(e1, s));
+    T* Bp = B.bvec(26);
+    T* Xp = B.bvec(27);
+    const int64_t nr = (int64_t)n * nrhs, npad = (int64_t)n * (kb - nrhs);
+    launch_copy<T>(s, nr, b, Bp);
+    launch_copy<T>(s, nr, x, Xp);
+    if (npad) {
+        launch_fill<T>(s, npad, Bp + nr, Host<T>::zero());
+        launch_fill<T>(s, npad, Xp + nr, Host<T>::zero());
+    }
+    std::vector<double> nB(kb, 0.0);
+    for (int j = 0; j < nrhs; ++j) nB[j] = normB[j];
+    auto col = B.run(conv_cri, max_ite, nB, Bp, Xp);
+    launch_copy<T>(s, nr, Xp, x);
+    // true residuals ||b_j - A x_j|| / normB_j
+    T* Rt = B.bvec(21);
+    launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), Xp, Rt,
+                        nullptr, nullptr);
+    launch_residual<T>(s, n * kb, Bp, Rt, Rt);
+    B.bpartial();
+    launch_norm2_cols<T>(s, kb, n, n, Rt, c->d_bpartial.as<double>());
+    launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
+    SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * kb, hipMemcpyDeviceToHost, s));
+    SSGPU_HIP(hipEventRecord(e2, s));
+    SSGPU_HIP(hipEventSynchronize(e2));
+    S.check_sweep_error();
+    float t_it = 0.f;
+    (void)hipEventElapsedTime(&t_it, e1, e2);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    int all_conv = 1;
+    for (int j = 0; j < nrhs; ++j) {
+        ssgpu_result& r = res[j];
+        r = ssgpu_result{};
+        r.converged = col[j].conv ? 1 : 0;
+        r.status = col[j].conv ? OK : NOT_CONVERGED;
+        r.iterations = col[j].iterations;
+        r.final_residual = col[j].final_res;
+        r.accel_used = m.factor_accel;
+        r.pivot_fixes = m.factor_pivot_fixes;
+        r.iterate_ms = t_it;
+        r.true_residual = std::sqrt(c->h_scal[j]) / (normB[j] > 0.0 ? normB[j] : 1.0);
+        all_conv &= r.converged;
+    }
+    return all_conv ? OK : NOT_CONVERGED;
+}
+
 // Build the preconditioner a cached solve uses: the shifted IC(0) factor
 // (automatic shift when accel < 0) or, for SGS-MRTR, S A S and its triangles.
 template <typename T> void factorize(Matrix& m, int solver, double accel, int precision, double* accel_used,
@@ -1186,6 +1465,7 @@ static void context_free(ssgpu_context ctx) {
     ctx->c.d_tickets.release();
     ctx->c.d_err.release();
     if (ctx->c.h_scal) (void)hipHostFree(ctx->c.h_scal);
+    if (ctx->c.h_bscal) (void)hipHostFree(ctx->c.h_bscal);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
@@ -1354,6 +1634,23 @@ int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix mh, int solver, const s
                                       (double*)d_x, log, log_cap, res);
     return solve_factored<zc>(c, m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log, log_cap,
                               res);
+    SSGPU_GUARD_END
+}
+
+int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
+                      int nrhs, const double* normB, const void* d_B, void* d_X, ssgpu_result* res) {
+    if (!ctx || !mh || !opt || !normB || !d_B || !d_X || !res || nrhs < 1 || nrhs > 32) return SSGPU_BAD_ARG;
+    const int ck = check_opts(opt, SSGPU_ICCG);
+    if (ck != SSGPU_OK) return ck;
+    SSGPU_GUARD_BEGIN
+    Matrix& m = mh->m;
+    Context* c = &ctx->c;
+    if (c->device != m.ctx->device) throw Error{BAD_ARG, "context and matrix are on different devices"};
+    std::lock_guard<std::mutex> lk(c->mu);
+    SSGPU_HIP(hipSetDevice(c->device));
+    if (m.dtype == SSGPU_R64)
+        return solve_block<double>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const double*)d_B, (double*)d_X, res);
+    return solve_block<zc>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const zc*)d_B, (zc*)d_X, res);
     SSGPU_GUARD_END
 }
 

@@ -124,6 +124,26 @@ template <typename T, typename F>
 void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, int max_idle, T* x);
+// block sweep over ncol right-hand sides (column c at offset c * n), kc in
+// {1, 2, 4 (real only)} columns per lane (registers: KMAX x kc inputs),
+// ncol a multiple of kc; tickets: ncol / kc zeroed counters
+template <typename T, typename F>
+void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
+                        const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
+                        const F* dval, const T* rhs, const T* aux, const T* dd, T* x);
+// --- block solves (ssgpu_block.hip): kb columns, column c at offset c * ld ------------
+template <typename T>
+void launch_spmv_cols(hipStream_t s, int kb, int ld, int nblk, const int* blk, const int* rp, const int* ci,
+                      const T* v, const T* x, T* y, const T* dotv, double* partial);
+// partials: kb * parts per block ((block * kb + c) * parts), finalized by launch_finalize
+template <typename T>
+void launch_dot_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, double* partial);
+template <typename T> void launch_norm2_cols(hipStream_t s, int kb, int n, int ld, const T* a, double* partial);
+template <typename T>
+void launch_cg_update_cols(hipStream_t s, int kb, int n, int ld, const T* alpha, const int* act, const T* p,
+                           const T* q, T* x, T* r, double* partial);
+template <typename T>
+void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta, const int* act, const T* z, T* p);
 // dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)
 template <typename T, typename F>
 void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);

@@ -180,6 +180,110 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
     }
 }
 
+// Several right-hand sides at once (block solves): x, rhs and aux hold ncol
+// vectors of n entries (entry (i, c) at c * n + i).  Workgroup (., g) sweeps
+// columns [g*KC, g*KC + KC) with its own ticket counter tickets[g]; a lane
+// keeps its row's entries once and KC inputs per entry.  Each column's
+// result is formed exactly as the one-column sweep forms it.
+template <typename T, typename F, int MODE, int KMAX, int KC>
+__global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int ncol, int* __restrict__ tickets,
+                                                         int* __restrict__ err, const int* __restrict__ srow,
+                                                         const int* __restrict__ slice_off,
+                                                         const int* __restrict__ rlen, const int* __restrict__ ecol,
+                                                         const F* __restrict__ eval, const F* __restrict__ dval,
+                                                         const T* __restrict__ rhs, const T* __restrict__ aux,
+                                                         const T* __restrict__ dd, T* __restrict__ x) {
+    const int lane = threadIdx.x;
+    const int c0 = (int)blockIdx.y * KC;
+    int* ticket = tickets + blockIdx.y;
+    long spins = 0;
+    while (true) {
+        const int c = take_ticket(ticket);
+        if (c >= nchunks) return;
+        const int p = c * kSfRows + lane;
+        const bool active = p < n;
+        const int i = active ? srow[p] : 0;
+        const int len = active ? rlen[p] : 0;
+        int cols[KMAX];
+        F vals[KMAX];
+        const int base = slice_off[c] + lane;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            if (k < len) {
+                cols[k] = ecol[base + kSfRows * k];
+                vals[k] = eval[base + kSfRows * k];
+            }
+        T rv[KC];
+        T d_i = Num<T>::zero();
+        F dg = F();
+#pragma unroll
+        for (int q = 0; q < KC; ++q) rv[q] = Num<T>::zero();
+        if (active) {
+            if (MODE == BWD_IC) {
+                d_i = dd[i];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) rv[q] = aux[(size_t)(c0 + q) * n + i];
+            } else {
+                dg = dval[p];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) rv[q] = rhs[(size_t)(c0 + q) * n + i];
+            }
+        }
+        T g[KMAX][KC];
+        unsigned miss = active ? (len >= 32 ? ~0u : ((1u << len) - 1u)) : 0u;
+        bool done = !active;
+        int idle = 0;
+        while (true) {
+            __asm__ __volatile__("" ::: "memory");
+            bool progress = false;
+            if (!done && miss) {
+                const unsigned want = miss;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (want & (1u << k)) {
+#pragma unroll
+                        for (int q = 0; q < KC; ++q) g[k][q] = gload(x + (size_t)(c0 + q) * n + cols[k]);
+                    }
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (want & (1u << k)) {
+                        bool ok = true;
+#pragma unroll
+                        for (int q = 0; q < KC; ++q) ok = ok && !sent(g[k][q]);
+                        if (ok) miss &= ~(1u << k);
+                    }
+                progress = miss != want;
+            }
+            if (!done && miss == 0u) {
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    T sum = (MODE == BWD_IC) ? Num<T>::zero() : rv[q];
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
+                        if (k < len) sum = Num<T>::sub(sum, Num<T>::mul(wid<T, F>(vals[k]), g[k][q]));
+                    const T out = (MODE == BWD_IC) ? Num<T>::add(Num<T>::mul(sum, d_i), rv[q])
+                                                   : Num<T>::div(sum, wid<T, F>(dg));
+                    gstore(x + (size_t)(c0 + q) * n + i, out);
+                }
+                done = true;
+                progress = true;
+            }
+            if (__all(done)) break;
+            if (++spins > kSpinLimit) {
+                atomicOr(err, 1);
+                return;
+            }
+            if (__any(progress)) {
+                idle = 0;
+            } else {
+                if (idle < 2) ++idle;
+                if (idle == 1) __builtin_amdgcn_s_sleep(1);
+                else __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_sf_prepare(int n, T* __restrict__ a, T* __restrict__ b, int* __restrict__ tickets) {
     const T s = sent_val((T*)nullptr);
@@ -252,6 +356,22 @@ void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks
 #undef SSGPU_SF
 }
 
+template <typename T, typename F, int MODE, int KC>
+void sweep_block_kmax(hipStream_t s, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
+                      const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
+                      const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+#define SSGPU_SB(K)                                                                                                \
+    hipLaunchKernelGGL((k_sweep_block<T, F, MODE, K, KC>), g, dim3(kSfRows), 0, s, n, nchunks, ncol, tickets, err, \
+                       srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+    if (kmax <= 8) SSGPU_SB(8);
+    else if (kmax <= 12) SSGPU_SB(12);
+    else if (kmax <= 16) SSGPU_SB(16);
+    else if (kmax <= 20) SSGPU_SB(20);
+    else if (kmax <= 24) SSGPU_SB(24);
+    else SSGPU_SB(32);
+#undef SSGPU_SB
+}
+
 }  // namespace
 
 template <typename T, typename F>
@@ -269,6 +389,38 @@ void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ti
     else
         sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
                                         eval, dval, rhs, aux, dd, x);
+}
+
+template <typename T, typename F, int KC>
+void sweep_block_kc(hipStream_t s, int mode, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
+                    const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
+                    const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+    if (mode == FWD_DIV_LAST)
+        sweep_block_kmax<T, F, FWD_DIV_LAST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen,
+                                                  ecol, eval, dval, rhs, aux, dd, x);
+    else if (mode == BWD_IC)
+        sweep_block_kmax<T, F, BWD_IC, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol,
+                                            eval, dval, rhs, aux, dd, x);
+    else
+        sweep_block_kmax<T, F, BWD_DIV_FIRST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen,
+                                                   ecol, eval, dval, rhs, aux, dd, x);
+}
+
+template <typename T, typename F>
+void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
+                        const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
+                        const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+    const int nchunks = (n + kSfRows - 1) / kSfRows;
+    const dim3 g((unsigned)std::max(1, std::min(grid, nchunks)), (unsigned)(ncol / kc));
+    if (kc == 4 && sizeof(T) == 8)
+        sweep_block_kc<T, F, 4>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
+                                dval, rhs, aux, dd, x);
+    else if (kc == 2)
+        sweep_block_kc<T, F, 2>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
+                                dval, rhs, aux, dd, x);
+    else
+        sweep_block_kc<T, F, 1>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
+                                dval, rhs, aux, dd, x);
 }
 
 template <typename T>
@@ -295,6 +447,18 @@ template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int
 template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
                                          const int*, const int*, const fc*, const fc*, const zc*, const zc*,
                                          const zc*, int, zc*);
+template void launch_sweep_block<double, double>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
+                                            const int*, const int*, const int*, const double*, const double*, const double*,
+                                            const double*, const double*, double*);
+template void launch_sweep_block<double, float>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
+                                            const int*, const int*, const int*, const float*, const float*, const double*,
+                                            const double*, const double*, double*);
+template void launch_sweep_block<zc, zc>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
+                                            const int*, const int*, const int*, const zc*, const zc*, const zc*,
+                                            const zc*, const zc*, zc*);
+template void launch_sweep_block<zc, fc>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
+                                            const int*, const int*, const int*, const fc*, const fc*, const zc*,
+                                            const zc*, const zc*, zc*);
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);

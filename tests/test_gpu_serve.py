@@ -124,3 +124,72 @@ def test_serving_mix_matches_solo(torch_cuda, ctx):
         x_solo, r_solo = _solo(torch_cuda, ctx, systems[key], b, solver, 1.1, prec)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo), (key, solver, prec)
+
+
+def _solo_factored(torch, ctx, m, b, opts):
+    d_b = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+    d_x = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    r = m.solve_factored(ctx, capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=5000,
+                         normB=float(np.linalg.norm(b)), opts=opts)
+    return d_x.cpu().numpy(), r
+
+
+@pytest.mark.parametrize("cplx,k", [(False, 1), (False, 3), (False, 8), (False, 32), (True, 2), (True, 5)])
+def test_block_solve_columns_match_single_solves(torch_cuda, ctx, cplx, k):
+    s = voxel.eddy_current(7, seed=6) if cplx else voxel.curlcurl(9, seed=6)
+    # columns of different difficulty: the loop-coil RHS, A u, a scaled copy, a zero column
+    cols = []
+    for j in range(k):
+        if j % 4 == 0:
+            cols.append(s.rhs * (1.0 + j))
+        elif j % 4 == 3:
+            cols.append(np.zeros_like(s.rhs))
+        else:
+            cols.append(_rhs(s, 50 + j))
+    B = np.stack(cols, axis=1)           # column j = right-hand side j
+    print(f"block cplx={cplx} k={k} N={s.n}", flush=True)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    m.factorize(capi.ICCG, 1.1)
+    opts = capi.default_options(accel_factor=1.1)
+    d_B = torch_cuda.from_numpy(np.ascontiguousarray(B.T)).cuda()   # vectors one after the other
+    d_X = torch_cuda.zeros_like(d_B)
+    torch_cuda.cuda.synchronize()
+    normB = [float(np.linalg.norm(B[:, j])) for j in range(k)]
+    c2 = capi.Context(0)
+    res = m.solve_block(c2, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=1e-8, max_ite=5000, opts=opts)
+    X = d_X.cpu().numpy().T
+    for j in range(k):
+        xs, rs = _solo_factored(torch_cuda, ctx, m, B[:, j], opts)
+        assert res[j].converged == rs.converged and res[j].iterations == rs.iterations, (j, res[j].iterations,
+                                                                                          rs.iterations)
+        assert np.array_equal(X[:, j], xs), j
+        assert res[j].final_residual == rs.final_residual
+    c2.close()
+    m.close()
+
+
+def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
+    s = voxel.curlcurl(9, seed=7)
+    k = 4
+    B = np.stack([_rhs(s, 70 + j) for j in range(k)], axis=1)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    m.factorize(capi.ICCG, 1.1, capi.FACTOR_F32)
+    for opts in (capi.default_options(accel_factor=1.1, factor_precision=capi.FACTOR_F32),
+                 capi.default_options(accel_factor=1.1, is_save_best=1, diverge_judge_type=1, bad_div_val=2.0,
+                                      bad_div_count_thres=3)):
+        d_B = torch_cuda.from_numpy(np.ascontiguousarray(B.T)).cuda()
+        d_X = torch_cuda.zeros_like(d_B)
+        torch_cuda.cuda.synchronize()
+        normB = [float(np.linalg.norm(B[:, j])) for j in range(k)]
+        res = m.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, max_ite=8, opts=opts)
+        X = d_X.cpu().numpy().T
+        for j in range(k):
+            d_b = torch_cuda.from_numpy(np.ascontiguousarray(B[:, j])).cuda()
+            d_x = torch_cuda.zeros_like(d_b)
+            torch_cuda.cuda.synchronize()
+            rs = m.solve_factored(ctx, capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), max_ite=8, normB=normB[j],
+                                  opts=opts)
+            assert res[j].iterations == rs.iterations and res[j].converged == rs.converged
+            assert np.array_equal(X[:, j], d_x.cpu().numpy()), j
+    m.close()
