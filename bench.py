@@ -80,7 +80,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 def served_run(a, s, code, release):
     """The same system solved for many right-hand sides through the serving
     engine: the matrix and its factor stay resident (built once, by the first
-    request), and `slots` solves run at once on separate streams.  Right-hand
+    request); requests queued together for it are batched into block solves
+    (up to 32 right-hand sides per pass of the kernels), and up to `slots`
+    solves run at once on separate streams.  Right-hand
     sides are the same physics with the coil moved (seeded slab height and
     centre), so each needs about as many iterations as the drop-in's solve.
     Measured with the FP32 and with the FP64 factor; latency counts from
@@ -90,6 +92,7 @@ def served_run(a, s, code, release):
     rhs = [voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering, coil_seed=k + 1).rhs
            for k in range(a.serve_requests)]
     out = {"slots": a.serve_slots, "requests": a.serve_requests, "rhs": "loop coil moved per request (seeded)",
+           "batches": {},
            "solves_per_s": {}, "iterations_per_s": {}, "latency_ms": {}, "iterations": {},
            "max_true_residual": {}, "setup_s": {}}
     eng = serve.Engine(slots=a.serve_slots)
@@ -101,14 +104,16 @@ def served_run(a, s, code, release):
             t0 = time.perf_counter()
             first = eng.submit(req(rhs[0])).result()   # builds the resident system and factor
             out["setup_s"][name] = time.perf_counter() - t0
+            eng.batches.clear()
             t0 = time.perf_counter()
-            futs = [eng.submit(req(b)) for b in rhs]
+            futs = eng.submit_many([req(b) for b in rhs])
             resp = [f.result() for f in futs]
             el = time.perf_counter() - t0
             bad = [r.error for r in resp if r.error] + ([first.error] if first.error else [])
             if bad:
                 raise RuntimeError(bad[0])
             out["solves_per_s"][name] = len(resp) / el
+            out["batches"][name] = list(eng.batches)
             out["iterations_per_s"][name] = sum(r.result.iterations for r in resp) / el
             out["latency_ms"][name] = {k: 1e3 * v for k, v in serve.percentiles([r.latency_s for r in resp]).items()}
             out["iterations"][name] = [min(r.result.iterations for r in resp), max(r.result.iterations for r in resp)]
@@ -169,7 +174,7 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=32)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU layout is design-only",

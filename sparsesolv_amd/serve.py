@@ -5,10 +5,15 @@ settings) are admitted as they arrive into a fixed number of slots.  Each
 slot is a worker thread with its own HIP stream (a library context), so the
 solves of different slots overlap on the device: a sweep keeps only a few
 hundred waves busy and mostly waits on its dependency chain, which leaves
-the rest of the GPU to the other slots.  Matrices stay resident with their
-preconditioner (the shifted IC(0) factor, or S A S for SGS-MRTR) in a cache
-keyed by (matrix key, solver family, shift, factor precision); a request for
-a cached system skips the upload, the analysis and the factorization.  The
+the rest of the GPU to the other slots.  Batching is continuous: a slot that
+takes an ICCG request also takes every queued request for the same resident
+system and settings (up to 32) and runs them as one block solve, whose
+sweeps carry all the right-hand sides through the dependency chain at once;
+requests that arrive while it runs wait for the next free slot.  Matrices
+stay resident with their preconditioner (the shifted IC(0) factor, or S A S
+for SGS-MRTR) in a cache keyed by (matrix key, solver family, shift, factor
+precision); a request for a cached system skips the upload, the analysis and
+the factorization.  The
 cache is bounded by an HBM budget: least recently used systems nobody is
 solving on are evicted to admit a new one, and a request waits while the
 budget is taken by systems in use.
@@ -23,7 +28,6 @@ The reference has no serving layer; its solvers refactor on every call
 from __future__ import annotations
 
 import collections
-import queue
 import threading
 import time
 from dataclasses import dataclass, field
@@ -67,10 +71,6 @@ def _system_bytes(nnz: int, n: int, cplx: bool) -> int:
     return int((95 if cplx else 60) * nnz + 64 * n)
 
 
-def _slot_bytes(n: int, cplx: bool) -> int:
-    return 16 * n * (16 if cplx else 8)   # Krylov work vectors of one slot
-
-
 class _Entry:
     def __init__(self, matrix: capi.Matrix, nbytes: int):
         self.matrix = matrix
@@ -80,8 +80,11 @@ class _Entry:
         self.error: Optional[str] = None
 
 
+MAX_BATCH = 32   # right-hand sides per block solve
+
+
 class Engine:
-    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: Optional[int] = None):
+    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: Optional[int] = None, batching: bool = True):
         import torch
         self.torch = torch
         self.device = device
@@ -92,7 +95,11 @@ class Engine:
         self.cache: "collections.OrderedDict[object, _Entry]" = collections.OrderedDict()
         self.resident = 0
         self.peak_resident = 0
-        self.q: "queue.Queue" = queue.Queue()
+        self.batching = batching
+        self.pending: "collections.deque" = collections.deque()
+        self.qcv = threading.Condition()
+        self.closing = False
+        self.batches = []          # sizes of the solves run (1 = single solve)
         self.workers = []
         for _ in range(slots):
             t = threading.Thread(target=self._worker, daemon=True)
@@ -105,17 +112,35 @@ class Engine:
         import concurrent.futures
         fut = concurrent.futures.Future()
         req.arrival = time.perf_counter()
-        self.q.put((req, fut))
+        with self.qcv:
+            self.pending.append((req, fut))
+            self.qcv.notify()
         return fut
 
+    def submit_many(self, reqs):
+        """Queue several requests at once (they arrive together)."""
+        import concurrent.futures
+        futs = []
+        now = time.perf_counter()
+        with self.qcv:
+            for r in reqs:
+                fut = concurrent.futures.Future()
+                r.arrival = now
+                self.pending.append((r, fut))
+                futs.append(fut)
+            self.qcv.notify_all()
+        return futs
+
     def close(self):
-        for _ in self.workers:
-            self.q.put(None)
+        with self.qcv:
+            self.closing = True
+            self.qcv.notify_all()
         for t in self.workers:
             t.join()
         with self.lock:
             for e in self.cache.values():
-                e.matrix.close()
+                if e.matrix is not None:
+                    e.matrix.close()
             self.cache.clear()
         self.home.close()
 
@@ -123,6 +148,33 @@ class Engine:
     def _cache_key(self, r: Request):
         family = "sgs" if r.solver == capi.MRTR_SGS else "ic"
         return (r.key, family, None if family == "sgs" else float(r.accel), int(r.precision))
+
+    def _batch_key(self, r: Request):
+        """Requests with equal keys can share one block solve."""
+        if not self.batching or r.solver != capi.ICCG:
+            return None
+        return (self._cache_key(r), float(r.conv_cri), int(r.max_ite), tuple(sorted(r.opts.items())))
+
+    def _take(self):
+        """Next request plus the queued ones it can be batched with; None at close."""
+        with self.qcv:
+            while not self.pending and not self.closing:
+                self.qcv.wait()
+            if not self.pending:
+                return None
+            first = self.pending.popleft()
+            batch = [first]
+            bk = self._batch_key(first[0])
+            if bk is not None:
+                keep = collections.deque()
+                while self.pending:
+                    item = self.pending.popleft()
+                    if len(batch) < MAX_BATCH and self._batch_key(item[0]) == bk:
+                        batch.append(item)
+                    else:
+                        keep.append(item)
+                self.pending = keep
+            return batch
 
     def _acquire(self, r: Request):
         """The resident system for r (built if needed), with a user reference."""
@@ -186,29 +238,40 @@ class Engine:
         ctx = capi.Context(self.device)
         stream = torch.cuda.Stream(device=self.device)
         while True:
-            item = self.q.get()
-            if item is None:
+            batch = self._take()
+            if batch is None:
                 break
-            r, fut = item
+            r0 = batch[0][0]
             try:
-                key, e, hit = self._acquire(r)
+                key, e, hit = self._acquire(r0)
             except Exception as ex:  # noqa: BLE001
-                fut.set_result(Response(None, None, time.perf_counter() - r.arrival, False, str(ex)))
+                for r, fut in batch:
+                    fut.set_result(Response(None, None, time.perf_counter() - r.arrival, False, str(ex)))
                 continue
             try:
+                opts = capi.default_options(accel_factor=r0.accel, factor_precision=r0.precision, **r0.opts)
                 with torch.cuda.stream(stream):
-                    d_b = torch.from_numpy(np.ascontiguousarray(r.rhs)).to("cuda", non_blocking=False)
+                    d_b = torch.from_numpy(np.ascontiguousarray(np.stack([r.rhs for r, _ in batch]))).to("cuda")
                     d_x = torch.zeros_like(d_b)
                 stream.synchronize()
-                normB = float(np.sqrt(np.sum(np.abs(r.rhs) ** 2)))
-                opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
-                res = e.matrix.solve_factored(ctx, r.solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=r.conv_cri,
-                                              max_ite=r.max_ite, normB=normB, opts=opts)
+                normB = [float(np.sqrt(np.sum(np.abs(r.rhs) ** 2))) for r, _ in batch]
+                if len(batch) == 1:
+                    results = [e.matrix.solve_factored(ctx, r0.solver, d_b.data_ptr(), d_x.data_ptr(),
+                                                       conv_cri=r0.conv_cri, max_ite=r0.max_ite, normB=normB[0],
+                                                       opts=opts)]
+                else:
+                    results = e.matrix.solve_block(ctx, d_b.data_ptr(), d_x.data_ptr(), normB, conv_cri=r0.conv_cri,
+                                                   max_ite=r0.max_ite, opts=opts)
                 with torch.cuda.stream(stream):
-                    x = d_x.cpu().numpy()
-                fut.set_result(Response(x, res, time.perf_counter() - r.arrival, hit))
+                    xs = d_x.cpu().numpy()
+                with self.lock:
+                    self.batches.append(len(batch))
+                done = time.perf_counter()
+                for i, (r, fut) in enumerate(batch):
+                    fut.set_result(Response(xs[i].copy(), results[i], done - r.arrival, hit))
             except Exception as ex:  # noqa: BLE001
-                fut.set_result(Response(None, None, time.perf_counter() - r.arrival, hit, str(ex)))
+                for r, fut in batch:
+                    fut.set_result(Response(None, None, time.perf_counter() - r.arrival, hit, str(ex)))
             finally:
                 self._release(key, e)
         ctx.close()

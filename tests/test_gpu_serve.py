@@ -193,3 +193,27 @@ def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
             assert res[j].iterations == rs.iterations and res[j].converged == rs.converged
             assert np.array_equal(X[:, j], d_x.cpu().numpy()), j
     m.close()
+
+
+def test_serving_batches_same_system_requests(torch_cuda, ctx):
+    """Requests queued together for one system run as one block solve; each
+    response still equals its solo solve."""
+    s = voxel.curlcurl(10, seed=8)
+    rhs = [_rhs(s, 200 + k) for k in range(9)]
+    eng = serve.Engine(slots=2)
+    first = eng.submit(serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=rhs[0],
+                                     accel=1.1)).result(timeout=120)
+    assert first.error is None
+    eng.batches.clear()
+    futs = eng.submit_many([serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                          accel=1.1) for b in rhs])
+    resp = [f.result(timeout=120) for f in futs]
+    batches = list(eng.batches)
+    eng.close()
+    print(f"batches {batches}", flush=True)
+    assert sum(batches) == len(rhs) and max(batches) > 1
+    for b, r in zip(rhs, resp):
+        assert r.error is None, r.error
+        x_solo, r_solo = _solo(torch_cuda, ctx, s, b, capi.ICCG, 1.1)
+        assert r.result.iterations == r_solo.iterations
+        assert np.array_equal(r.x, x_solo)
