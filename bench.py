@@ -158,6 +158,119 @@ def traffic_for(n, kernel):
         return None
 
 
+def _engine_run(eng, reqs, arrivals=None):
+    """Submit reqs (all at once, or at the given offsets in seconds) and wait;
+    returns (responses, elapsed seconds from the first arrival)."""
+    t0 = time.perf_counter()
+    if arrivals is None:
+        futs = eng.submit_many(reqs)
+    else:
+        futs = []
+        for r, at in zip(reqs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(eng.submit(r))
+    resp = [f.result() for f in futs]
+    return resp, time.perf_counter() - t0
+
+
+def _report(metric, resp, el, eng, extra):
+    from sparsesolv_amd import serve
+    bad = [r.error for r in resp if r.error]
+    if bad:
+        raise RuntimeError(bad[0])
+    lat = serve.percentiles([r.latency_s for r in resp])
+    out = {"metric": metric, "value": len(resp) / el, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+           "elapsed_s": el, "requests": len(resp),
+           "latency_ms": {k: 1e3 * v for k, v in lat.items()},
+           "converged": sum(bool(r.result.converged) for r in resp),
+           "max_true_residual": max(r.result.true_residual for r in resp),
+           "cache_hits": sum(r.cache_hit for r in resp),
+           "batch_sizes": {str(k): eng.batches.count(k) for k in sorted(set(eng.batches))},
+           "peak_resident_bytes": eng.peak_resident, "data": "synthetic (seeded voxel systems)"}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def workload_batched(a):
+    """256 independent curl-curl systems (grid edge uniform in 8..40, seeded
+    iron layouts, shifts uniform in [1.0, 1.3]) and one 60^3 system with 32
+    consistent right-hand sides (A u, u uniform in [-1, 1]) sharing one
+    factor, all submitted at once to the serving engine; FP64 factors, then
+    FP32 factors."""
+    from sparsesolv_amd import capi, serve, voxel
+    rng = np.random.default_rng(4)
+    systems = []
+    for k in range(256):
+        n = int(rng.integers(8, 41))
+        systems.append((f"b{k}", voxel.curlcurl(n, seed=1000 + k, iron_fraction=0.2), float(rng.uniform(1.0, 1.3))))
+    big = voxel.curlcurl(60, seed=7, iron_fraction=0.2)
+    A = big.scipy()
+    big_rhs = [A @ np.random.default_rng(500 + j).uniform(-1.0, 1.0, big.n) for j in range(32)]
+    print(f"# batched: 256 systems, N {min(s.n for _, s, _ in systems)}..{max(s.n for _, s, _ in systems)} "
+          f"(sum {sum(s.n for _, s, _ in systems)}), plus N={big.n} x 32 rhs", file=sys.stderr, flush=True)
+    for name, prec in (("f64", capi.FACTOR_F64), ("f32_factor", capi.FACTOR_F32)):
+        reqs = [serve.Request(key=(key, name), row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=s.rhs,
+                              accel=acc, precision=prec) for key, s, acc in systems]
+        reqs += [serve.Request(key=("big", name), row_ptr=big.row_ptr, col_idx=big.col_idx, values=big.values, rhs=b,
+                               accel=1.15, precision=prec) for b in big_rhs]
+        eng = serve.Engine(slots=a.serve_slots)
+        try:
+            resp, el = _engine_run(eng, reqs)
+            print(f"# batched {name}: {len(resp)} solves in {el:.2f}s", file=sys.stderr, flush=True)
+            _report("batched solves/s (256 systems n=8..40 + 60^3 x 32 rhs)", resp, el, eng,
+                    {"factor": name, "slots": a.serve_slots,
+                     "config": {"workload": "256 independent systems n~U{8..40} + one 60^3 system x 32 rhs",
+                                "includes": "upload, analysis and factorization of every system"}})
+        finally:
+            eng.close()
+
+
+def workload_serving(a):
+    """Poisson arrivals (rate a.rate per second) of curl-curl solve requests:
+    grid edge log-uniform in 10..120, 70% real magnetostatic / 30% complex
+    eddy-current, 40% reusing an earlier system with a new right-hand side
+    (loop coil moved); mixed-precision (FP32-factor) ICCG/COCG to 1e-8."""
+    from sparsesolv_amd import capi, serve, voxel
+    rng = np.random.default_rng(11)
+    seen, reqs = [], []
+    cache = {}
+    for k in range(a.requests):
+        if seen and rng.uniform() < 0.4:
+            key, n, cplx = seen[int(rng.integers(len(seen)))]
+            seed = 100000 + k
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(10), math.log(120)))))
+            cplx = rng.uniform() < 0.3
+            key = (k, n, cplx)
+            seen.append((key, n, cplx))
+            seed = None
+        if key not in cache:
+            cache[key] = (voxel.eddy_current(n, seed=key[0]) if cplx
+                          else voxel.curlcurl(n, seed=key[0], iron_fraction=0.2))
+        s = cache[key]
+        if seed is None or cplx:
+            rhs = s.rhs if seed is None else s.scipy() @ np.random.default_rng(seed).uniform(-1, 1, s.n)
+        else:
+            rhs = voxel.curlcurl(n, seed=key[0], iron_fraction=0.2, coil_seed=seed).rhs
+        reqs.append(serve.Request(key=key, row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=rhs,
+                                  accel=1.1, precision=capi.FACTOR_F32))
+    arrivals = np.cumsum(rng.exponential(1.0 / a.rate, size=len(reqs)))
+    sizes = [len(r.row_ptr) - 1 for r in reqs]
+    print(f"# serving: {len(reqs)} requests at {a.rate}/s, N {min(sizes)}..{max(sizes)}, "
+          f"{len(cache)} distinct systems", file=sys.stderr, flush=True)
+    eng = serve.Engine(slots=a.serve_slots)
+    try:
+        resp, el = _engine_run(eng, reqs, arrivals)
+        _report("served solves/s + p50/p99 latency (Poisson arrivals, mixed sizes and types)", resp, el, eng,
+                {"rate_per_s": a.rate, "slots": a.serve_slots, "distinct_systems": len(cache),
+                 "config": {"workload": "n log-uniform 10..120, 70% real / 30% complex, 40% repeat, FP32 factor",
+                            "includes": "upload, analysis and factorization on cache misses"}})
+    finally:
+        eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +288,11 @@ def main():
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving"],
+                    help="dropin: the n=95 solve (default); batched: 256 systems n=8..40 plus one n=60 system with "
+                         "32 right-hand sides; serving: Poisson arrivals of mixed sizes / types")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
+    ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU layout is design-only",
@@ -185,6 +303,10 @@ def main():
     from sparsesolv_amd import capi, voxel
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.workload == "batched":
+        return workload_batched(a)
+    if a.workload == "serving":
+        return workload_serving(a)
     t0 = time.perf_counter()
     s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
     t_gen = time.perf_counter() - t0

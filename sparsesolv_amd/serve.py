@@ -88,7 +88,6 @@ class Engine:
         import torch
         self.torch = torch
         self.device = device
-        self.home = capi.Context(device)          # owns the resident matrices
         total = torch.cuda.get_device_properties(device).total_memory
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * total)
         self.lock = threading.Condition()
@@ -142,7 +141,6 @@ class Engine:
                 if e.matrix is not None:
                     e.matrix.close()
             self.cache.clear()
-        self.home.close()
 
     # ------------------------------------------------------------ internals
     def _cache_key(self, r: Request):
@@ -176,8 +174,10 @@ class Engine:
                 self.pending = keep
             return batch
 
-    def _acquire(self, r: Request):
-        """The resident system for r (built if needed), with a user reference."""
+    def _acquire(self, r: Request, ctx: capi.Context):
+        """The resident system for r (built on the caller's context and stream
+        if needed, so misses in different slots build in parallel), with a user
+        reference."""
         key = self._cache_key(r)
         cplx = np.iscomplexobj(r.values)
         need = _system_bytes(len(r.values), len(r.row_ptr) - 1, cplx)
@@ -210,7 +210,7 @@ class Engine:
                 self.lock.wait()   # budget held by systems in use
         if not hit:
             try:
-                m = capi.Matrix(self.home, r.row_ptr, r.col_idx, r.values)
+                m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values)
                 m.factorize(r.solver, r.accel, r.precision)
                 e.matrix = m
             except Exception as ex:   # noqa: BLE001 - reported to every waiter
@@ -243,7 +243,7 @@ class Engine:
                 break
             r0 = batch[0][0]
             try:
-                key, e, hit = self._acquire(r0)
+                key, e, hit = self._acquire(r0, ctx)
             except Exception as ex:  # noqa: BLE001
                 for r, fut in batch:
                     fut.set_result(Response(None, None, time.perf_counter() - r.arrival, False, str(ex)))
