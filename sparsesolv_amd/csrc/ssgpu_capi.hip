@@ -216,37 +216,47 @@ static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const
     upload(c, out.map, map);
 }
 
-// rows: per row, off-diagonal (col, src_f, src_a) in summation order; diag (src_f, src_a)
-struct SfRow {
-    std::vector<int> col, sf, sa;
-    int df = -1, da = -1;
+// Per original row: its off-diagonal entries (col, source in the factor,
+// source in A) in summation order, and its diagonal's sources; flat arrays
+// with row offsets (entries of row i at off[i] .. off[i+1]).
+struct SfRows {
+    std::vector<int> off, col, sf, sa, df, da;
+    explicit SfRows(int n) : off(n + 1, 0), df(n, -1), da(n, -1) {}
+    void alloc() {
+        for (size_t i = 0; i + 1 < off.size(); ++i) off[i + 1] += off[i];
+        col.resize(off.back());
+        sf.resize(off.back());
+        sa.resize(off.back());
+    }
+    int len(int i) const { return off[i + 1] - off[i]; }
 };
-static void build_sf(Context* c, int n, const std::vector<int>& order, const std::vector<SfRow>& rows,
-                     const LevelCsr& lv, SfCsr& out) {
+
+static void build_sf(Context* c, int n, const std::vector<int>& order, const SfRows& rows, const LevelCsr& lv,
+                     SfCsr& out) {
     int kmax = 0;
-    for (const auto& r : rows) kmax = std::max(kmax, (int)r.col.size());
+    for (int i = 0; i < n; ++i) kmax = std::max(kmax, rows.len(i));
     const int nsl = (n + 63) / 64;
     std::vector<int> soff(nsl + 1, 0), rlen(n);
     for (int sl = 0; sl < nsl; ++sl) {
         int w = 0;
-        for (int p = 64 * sl; p < std::min(n, 64 * sl + 64); ++p) w = std::max(w, (int)rows[order[p]].col.size());
+        for (int p = 64 * sl; p < std::min(n, 64 * sl + 64); ++p) w = std::max(w, rows.len(order[p]));
         soff[sl + 1] = soff[sl] + 64 * w;
     }
     const int64_t nell = soff[nsl];
-
     std::vector<int> ecol(std::max<int64_t>(nell, 1), 0), ef(std::max<int64_t>(nell, 1), -1),
         ea(std::max<int64_t>(nell, 1), -1), df(n), da(n);
     for (int p = 0; p < n; ++p) {
-        const SfRow& r = rows[order[p]];
-        rlen[p] = (int)r.col.size();
+        const int i = order[p];
+        const int a = rows.off[i], len = rows.len(i);
+        rlen[p] = len;
         const int base = soff[p / 64] + (p % 64);
-        for (size_t k = 0; k < r.col.size(); ++k) {
-            ecol[base + 64 * k] = r.col[k];
-            ef[base + 64 * k] = r.sf[k];
-            ea[base + 64 * k] = r.sa[k];
+        for (int k = 0; k < len; ++k) {
+            ecol[base + 64 * k] = rows.col[a + k];
+            ef[base + 64 * k] = rows.sf[a + k];
+            ea[base + 64 * k] = rows.sa[a + k];
         }
-        df[p] = r.df;
-        da[p] = r.da;
+        df[p] = rows.df[i];
+        da[p] = rows.da[i];
     }
     out.kmax = kmax <= 64 ? std::max(kmax, 1) : 0;
     out.nell = nell;
@@ -258,6 +268,41 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const std
     upload(c, out.dmap_f, df);
     upload(c, out.emap_a, ea);
     upload(c, out.dmap_a, da);
+}
+
+// upper(A) (SprsTrans.cpp:387-394: entries col >= row, diagonal first) in
+// backward level order and its sliced-ELL copy, for the SGS sweeps; built
+// on first use
+static void ensure_upper(Matrix& m) {
+    if (m.us_built) return;
+    Context* c = m.ctx;
+    const int n = m.n;
+    const auto& rp = m.h_rp;
+    const auto& ci = m.h_ci;
+    std::vector<int> urp(n + 1, 0), uci, usrc, order_s;
+    m.upper_diag_ok = true;
+    for (int i = 0; i < n; ++i) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] >= i) { uci.push_back(ci[k]); usrc.push_back(k); }
+        urp[i + 1] = (int)uci.size();
+        if (urp[i + 1] == urp[i] || uci[urp[i]] != i) m.upper_diag_ok = false;
+    }
+    build_level_csr(c, n, urp, uci, usrc, false, m.Us, nullptr, &order_s);
+    SfRows rs(n);
+    for (int i = 0; i < n; ++i) rs.off[i + 1] = (urp[i + 1] - urp[i]) - ((urp[i + 1] > urp[i] && uci[urp[i]] == i) ? 1 : 0);
+    rs.alloc();
+    for (int i = 0; i < n; ++i) {
+        int q = rs.off[i];
+        for (int k = urp[i]; k < urp[i + 1]; ++k) {
+            if (k == urp[i] && uci[k] == i) { rs.da[i] = usrc[k]; continue; }
+            rs.col[q] = uci[k];
+            rs.sf[q] = -1;
+            rs.sa[q] = usrc[k];
+            ++q;
+        }
+    }
+    build_sf(c, n, order_s, rs, m.Us, m.sfUs);
+    m.us_built = true;
 }
 
 static void analyse(Matrix& m) {
@@ -300,7 +345,7 @@ static void analyse(Matrix& m) {
     }
     m.lmax_row = 0;
     for (int i = 0; i < n; ++i) m.lmax_row = std::max(m.lmax_row, lrp[i + 1] - lrp[i]);
-    std::vector<int> pos, order_l, order_u, order_s;
+    std::vector<int> pos, order_l, order_u;
     build_level_csr(c, n, lrp, lci, lsrc, true, m.L, &pos, &order_l);
     upload(c, m.ppos, pos);
     // transpose(lower) (SprsTrans.cpp:312-319): row j holds (j, i) for lower entries (i, j),
@@ -331,44 +376,35 @@ static void analyse(Matrix& m) {
         build_level_csr(c, n, urp, uci, usrc, false, m.Uic, nullptr, &order_u);
         m.uic_built = true;
         // sync-free copies: lower(A) (off-diagonals, diagonal = last entry) and transpose(lower)
-        std::vector<SfRow> rl(n), ru(n);
-        for (int i = 0; i < n; ++i) {
+        SfRows rl(n), ru(n);
+        for (int i = 0; i < n; ++i)
+            for (int k = lrp[i]; k < lrp[i + 1]; ++k)
+                if (lci[k] < i) {
+                    rl.off[i + 1]++;
+                    ru.off[lci[k] + 1]++;
+                }
+        rl.alloc();
+        ru.alloc();
+        std::vector<int> fl(rl.off.begin(), rl.off.end() - 1), fu(ru.off.begin(), ru.off.end() - 1);
+        for (int i = 0; i < n; ++i) {   // ascending i: each transposed row fills in ascending order
             const int a = lrp[i], e = lrp[i + 1];
             for (int k = a; k < e; ++k) {
                 const int j = lci[k];
                 const int f = lstart[i] + (k - a);
                 if (j < i) {
-                    rl[i].col.push_back(j); rl[i].sf.push_back(f); rl[i].sa.push_back(lsrc[k]);
-                    ru[j].col.push_back(i); ru[j].sf.push_back(f); ru[j].sa.push_back(-1);
+                    const int ql = fl[i]++, qu = fu[j]++;
+                    rl.col[ql] = j; rl.sf[ql] = f; rl.sa[ql] = lsrc[k];
+                    ru.col[qu] = i; ru.sf[qu] = f; ru.sa[qu] = -1;
                 } else if (j == i && k == e - 1) {
-                    rl[i].df = f; rl[i].da = lsrc[k];
-                    ru[i].df = f;
+                    rl.df[i] = f; rl.da[i] = lsrc[k];
+                    ru.df[i] = f;
                 }
             }
         }
         build_sf(c, n, order_l, rl, m.L, m.sfL);
         build_sf(c, n, order_u, ru, m.Uic, m.sfUic);
     }
-    // upper(A): entries col >= row (SprsTrans.cpp:387-394), for the SGS sweeps
-    {
-        std::vector<int> urp(n + 1, 0), uci, usrc;
-        m.upper_diag_ok = true;
-        for (int i = 0; i < n; ++i) {
-            for (int k = rp[i]; k < rp[i + 1]; ++k)
-                if (ci[k] >= i) { uci.push_back(ci[k]); usrc.push_back(k); }
-            urp[i + 1] = (int)uci.size();
-            if (urp[i + 1] == urp[i] || uci[urp[i]] != i) m.upper_diag_ok = false;
-        }
-        build_level_csr(c, n, urp, uci, usrc, false, m.Us, nullptr, &order_s);
-        m.us_built = true;
-        std::vector<SfRow> rs(n);
-        for (int i = 0; i < n; ++i)
-            for (int k = urp[i]; k < urp[i + 1]; ++k) {
-                if (k == urp[i] && uci[k] == i) { rs[i].da = usrc[k]; continue; }
-                rs[i].col.push_back(uci[k]); rs[i].sf.push_back(-1); rs[i].sa.push_back(usrc[k]);
-            }
-        build_sf(c, n, order_s, rs, m.Us, m.sfUs);
-    }
+    // upper(A) for the SGS sweeps: built on first use (ensure_upper)
 }
 
 // ------------------------------------------------------------------ helpers
@@ -943,6 +979,7 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
     double accel_used = o.accel_factor;
     int fixes = 0;
     if (solver == SSGPU_MRTR_SGS) {
+        ensure_upper(m);
         if (!m.lower_diag_ok || !m.upper_diag_ok)
             throw Error{NO_DIAGONAL, "SGS-MRTR needs every row's diagonal entry"};
         scaled = true;
@@ -1357,6 +1394,7 @@ template <typename T> void factorize(Matrix& m, int solver, double accel, int pr
     hipStream_t s = m.ctx->stream;
     const int n = m.n;
     if (solver == SSGPU_MRTR_SGS) {
+        ensure_upper(m);
         if (!m.lower_diag_ok || !m.upper_diag_ok)
             throw Error{NO_DIAGONAL, "SGS-MRTR needs every row's diagonal entry"};
         if (m.sc.bytes < sizeof(T) * n) m.sc.alloc(m.ctx, sizeof(T) * n);
@@ -1758,6 +1796,7 @@ static int plain_sweep(ssgpu_matrix mh, const void* d_r, void* d_z, bool lower) 
     Matrix& m = mh->m;
     Context* c = m.ctx;
     SSGPU_HIP(hipSetDevice(c->device));
+    ensure_upper(m);
     if (lower ? !m.lower_diag_ok : !m.upper_diag_ok) throw Error{NO_DIAGONAL, "missing diagonal entry"};
     ssgpu_options o;
     ssgpu_options_default(&o);
