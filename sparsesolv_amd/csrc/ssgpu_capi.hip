@@ -138,6 +138,7 @@ struct Matrix {
     int lmax_row = 0;          // longest row of lower(A)
     bool icsym_built = false;
     int64_t icsym_total = 0;
+    int ic_max_updates = 0;    // largest update list of a row
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
     int sweep_grid = 128;     // persistent one-wave workgroups per sweep
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
@@ -639,6 +640,8 @@ struct Solver {
         m.ic_col.alloc(c, sizeof(int) * std::max<size_t>((size_t)total, 1));
         launch_ic_symbolic_fill(s, m.n, m.L.prp.as<int>(), m.L.pci.as<int>(), m.L.prow.as<int>(), m.ppos.as<int>(),
                                 m.ic_po.as<int>(), m.ic_slot.p, m.ic_pos.as<int>(), m.ic_col.as<int>());
+        if (c->d_count.bytes == 0) c->d_count.alloc(c, 64);
+        m.ic_max_updates = ic_row_updates_max(s, m.n, m.L.prp.as<int>(), m.ic_po.as<int>(), c->d_count.as<int>());
         return true;
     }
     // Factor the matrix with values av (A or the scaled copy); returns pivot fixes.
@@ -652,11 +655,19 @@ struct Solver {
         launch_fill<T>(s, m.n, m.diagD.as<T>(), Host<T>::zero());
         launch_fill<int>(s, 1, c->d_count.as<int>(), 0);
         if (m.lmax_row <= 48 && ic_symbolic()) {
-            for (int l = 0; l < m.L.nlev; ++l)
-                launch_ic_numeric<T>(s, m.lmax_row, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(),
-                                     m.L.pci.as<int>(), m.L.prow.as<int>(), m.ic_po.as<int>(), m.ic_slot.p,
-                                     m.ic_pos.as<int>(), m.ic_col.as<int>(), m.Lv.as<T>(), m.diagD.as<T>(), accel,
-                                     c->d_count.as<int>());
+            launch_fill<int>(s, 1, c->d_count.as<int>(), 0);
+            for (int l = 0; l < m.L.nlev; ++l) {
+                if (m.ic_max_updates <= 256)   // wave per row: one gather round trip per row
+                    launch_ic_numeric_wave<T>(s, m.lmax_row, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(),
+                                              m.L.pci.as<int>(), m.L.prow.as<int>(), m.ic_po.as<int>(), m.ic_slot.p,
+                                              m.ic_pos.as<int>(), m.ic_col.as<int>(), m.Lv.as<T>(), m.diagD.as<T>(),
+                                              accel, c->d_count.as<int>());
+                else
+                    launch_ic_numeric<T>(s, m.lmax_row, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(),
+                                         m.L.pci.as<int>(), m.L.prow.as<int>(), m.ic_po.as<int>(), m.ic_slot.p,
+                                         m.ic_pos.as<int>(), m.ic_col.as<int>(), m.Lv.as<T>(), m.diagD.as<T>(), accel,
+                                         c->d_count.as<int>());
+            }
         } else {
             for (int l = 0; l < m.L.nlev; ++l)
                 launch_ic_level<T>(s, m.L.levptr[l], m.L.levptr[l + 1], m.L.prp.as<int>(), m.L.pci.as<int>(),

@@ -72,6 +72,8 @@ __global__ __launch_bounds__(kBS) void k_ic_symbolic(int n, const int* __restric
 }
 
 __device__ inline double absval(double a) { return fabs(a); }
+__device__ inline double shfl_idx(double v, int src) { return __shfl(v, src, 64); }
+__device__ inline zc shfl_idx(zc v, int src) { return zc{__shfl(v.re, src, 64), __shfl(v.im, src, 64)}; }
 __device__ inline double absval(zc a) { return hypot(a.re, a.im); }
 __device__ inline double mulr(double v, double a) { return v * a; }
 __device__ inline zc mulr(zc v, double a) { return v * a; }
@@ -130,7 +132,88 @@ __global__ __launch_bounds__(kFactBS) void k_ic_numeric(int p0, int p1, const in
     }
 }
 
+// Wave per row: the row's update list (at most 64 * kWaveUpd entries) and
+// its gathers of L_jk and d_k are loaded by the lanes at once (one memory
+// round trip per row instead of one per entry); then, entry by entry, the
+// lanes holding that entry's updates form their products in parallel and
+// lane 0 subtracts them in list order, so the factor is bit-identical to the
+// thread-per-row kernel's.  One wave per workgroup (its barriers are wave
+// barriers).
+constexpr int kWaveUpd = 4;   // updates per lane
+
+template <typename T, int KROW>
+__global__ __launch_bounds__(64) void k_ic_numeric_wave(int p0, int p1, const int* __restrict__ prp,
+                                                        const int* __restrict__ pci, const int* __restrict__ prow,
+                                                        const int* __restrict__ po,
+                                                        const unsigned char* __restrict__ u_slot,
+                                                        const int* __restrict__ u_pos, const int* __restrict__ u_col,
+                                                        T* __restrict__ fv, T* __restrict__ diagD, double accel,
+                                                        int* __restrict__ nfix) {
+    __shared__ T own_s[KROW];            // the row's finished entries
+    __shared__ T prod_s[64 * kWaveUpd];  // products of the current entry's updates
+    const int p = p0 + blockIdx.x;
+    if (p >= p1) return;
+    const int lane = threadIdx.x;
+    const int i = prow[p];
+    const int rs = prp[p], re = prp[p + 1];
+    const int ub = po[rs];
+    const int nu = po[re] - ub;
+    // gathers of the whole row, issued together
+    int sl[kWaveUpd];
+    T lj[kWaveUpd], dk[kWaveUpd];
+#pragma unroll
+    for (int q = 0; q < kWaveUpd; ++q) {
+        const int t = q * 64 + lane;
+        if (t < nu) {
+            sl[q] = u_slot[ub + t];
+            lj[q] = fv[u_pos[ub + t]];
+            dk[q] = diagD[u_col[ub + t]];
+        }
+    }
+    T a = Num<T>::zero();
+    if (lane < re - rs) a = fv[rs + lane];
+    for (int jj = rs; jj < re; ++jj) {
+        const int j = pci[jj];
+        const int t0 = po[jj] - ub, t1 = po[jj + 1] - ub;
+        // products of this entry's updates (own-row factors are final: k < j)
+#pragma unroll
+        for (int q = 0; q < kWaveUpd; ++q) {
+            const int t = q * 64 + lane;
+            if (t >= t0 && t < t1) {
+                const T lik = own_s[sl[q]];
+                prod_s[t - t0] = Num<T>::mul(Num<T>::mul(lik, (j == i) ? lik : lj[q]), dk[q]);
+            }
+        }
+        const T aj = shfl_idx(a, jj - rs);
+        __syncthreads();
+        if (lane == 0) {
+            T sum = (j == i) ? mulr(aj, accel) : aj;
+            for (int t = 0; t < t1 - t0; ++t) sum = Num<T>::sub(sum, prod_s[t]);   // SprsPreconditioner.cpp:75 / :98
+            fv[jj] = sum;
+            own_s[jj - rs] = sum;
+            if (j == i) {
+                if (absval(sum) < 1.0e-12) {
+                    diagD[i] = from_real<T>(1.0 / DBL_EPSILON);
+                    atomicAdd(nfix, 1);
+                } else {
+                    diagD[i] = recip(sum);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// largest update count of a row (the wave kernel's capacity check)
+__global__ void k_row_updates_max(int n, const int* __restrict__ prp, const int* __restrict__ po,
+                                  int* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) atomicMax(out, po[prp[p + 1]] - po[prp[p]]);
+}
+
 inline int grid_of(int64_t n, int bs) { return (int)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+__global__ void k_fill_one_int(int* p) { *p = 0; }
 
 }  // namespace
 
@@ -171,6 +254,35 @@ void launch_ic_numeric(hipStream_t s, int kmax_row, int p0, int p1, const int* p
         hipLaunchKernelGGL((k_ic_numeric<T, 48>), g, dim3(kFactBS), 0, s, p0, p1, prp, pci, prow, po, sl, u_pos, u_col,
                            fv, diagD, accel, nfix);
 }
+
+int ic_row_updates_max(hipStream_t s, int n, const int* prp, const int* po, int* scratch) {
+    hipLaunchKernelGGL(k_fill_one_int, dim3(1), dim3(1), 0, s, scratch);
+    hipLaunchKernelGGL(k_row_updates_max, dim3(grid_of(n, kBS)), dim3(kBS), 0, s, n, prp, po, scratch);
+    int h = 0;
+    (void)hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    return h;
+}
+
+template <typename T>
+void launch_ic_numeric_wave(hipStream_t s, int kmax_row, int p0, int p1, const int* prp, const int* pci,
+                            const int* prow, const int* po, const void* u_slot, const int* u_pos, const int* u_col,
+                            T* fv, T* diagD, double accel, int* nfix) {
+    const int nr = p1 - p0;
+    if (nr <= 0) return;
+    const unsigned char* sl = (const unsigned char*)u_slot;
+    if (kmax_row <= 24)
+        hipLaunchKernelGGL((k_ic_numeric_wave<T, 24>), dim3(nr), dim3(64), 0, s, p0, p1, prp, pci, prow, po, sl,
+                           u_pos, u_col, fv, diagD, accel, nfix);
+    else
+        hipLaunchKernelGGL((k_ic_numeric_wave<T, 48>), dim3(nr), dim3(64), 0, s, p0, p1, prp, pci, prow, po, sl,
+                           u_pos, u_col, fv, diagD, accel, nfix);
+}
+template void launch_ic_numeric_wave<double>(hipStream_t, int, int, int, const int*, const int*, const int*,
+                                             const int*, const void*, const int*, const int*, double*, double*,
+                                             double, int*);
+template void launch_ic_numeric_wave<zc>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
+                                         const void*, const int*, const int*, zc*, zc*, double, int*);
 
 template void launch_ic_numeric<double>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
                                         const void*, const int*, const int*, double*, double*, double, int*);

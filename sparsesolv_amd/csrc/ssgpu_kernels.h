@@ -55,6 +55,15 @@ void launch_ic_numeric(hipStream_t s, int kmax_row, int p0, int p1, const int* p
                        const int* po, const void* u_slot, const int* u_pos, const int* u_col, T* fv, T* diagD,
                        double accel, int* nfix);
 
+// wave-per-row variant (rows of at most 64 * 4 updates): same factor, one
+// gather round trip per row
+template <typename T>
+void launch_ic_numeric_wave(hipStream_t s, int kmax_row, int p0, int p1, const int* prp, const int* pci,
+                            const int* prow, const int* po, const void* u_slot, const int* u_pos, const int* u_col,
+                            T* fv, T* diagD, double accel, int* nfix);
+// largest update count of a row (synchronizes; scratch: one device int)
+int ic_row_updates_max(hipStream_t s, int n, const int* prp, const int* po, int* scratch);
+
 // --- reductions -----------------------------------------------------------------
 // Grid size used for n elements (fixed function of n).
 int reduce_blocks(int64_t n);
