@@ -271,6 +271,56 @@ def workload_serving(a):
         eng.close()
 
 
+def workload_eddy(a):
+    """Complex symmetric eddy-current system K + j w sigma M on a 64^3 voxel
+    grid (sigma = 5.8e7 S/m in a seeded conductor block, w = 2 pi 50), shifted
+    COCG at shift 1.1 to 1e-8: FP64 factor, then the complex64 factor with
+    FP32 sweeps (SpMV, transpose dots, recurrences and the true residual in
+    FP64).  Reports both and the FP32 result's distance to the FP64 one."""
+    import torch
+    from sparsesolv_amd import capi, voxel
+    n = 64 if a.n == 95 else a.n
+    s = voxel.eddy_current(n, seed=0)
+    ctx = capi.Context(0)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    normB = float(np.linalg.norm(s.rhs))
+    d_b = torch.from_numpy(s.rhs).cuda()
+    out = {"metric": "complex eddy-current COCG solves/s, FP64 vs complex64 factor", "unit": "solves/s",
+           "n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded voxel eddy-current system)",
+           "config": {"workload": f"eddy-current curl-curl + j w sigma M, n={n}^3, N={s.n}, nnz={s.nnz}, "
+                                  f"shift 1.1, tol 1e-8"}}
+    xs = {}
+    for name, prec in (("f64", capi.FACTOR_F64), ("c64_factor", capi.FACTOR_F32)):
+        opts = capi.default_options(accel_factor=1.1, factor_precision=prec)
+        d_x = torch.zeros_like(d_b)
+        torch.cuda.synchronize()
+        r = m.solve_device(capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=20000, normB=normB,
+                           opts=opts)   # warm
+        lat = []
+        for _ in range(max(1, a.steps)):
+            d_x.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = m.solve_device(capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=20000,
+                               normB=normB, opts=opts)
+            lat.append(time.perf_counter() - t)
+        xs[name] = d_x.cpu().numpy()
+        out[name] = {"solves_per_s": 1.0 / statistics.median(lat), "ms_per_solve": 1e3 * statistics.median(lat),
+                     "iterations": r.iterations, "converged": bool(r.converged), "true_residual": r.true_residual,
+                     "refinements": r.refinements, "setup_ms": r.setup_ms}
+        print(f"# eddy {name}: {1e3 * statistics.median(lat):.1f} ms, {r.iterations} iterations, "
+              f"true residual {r.true_residual:.2e}", file=sys.stderr, flush=True)
+    A = s.scipy()
+    out["c64_vs_f64"] = {"rel_residual_of_difference": float(np.linalg.norm(A @ (xs["c64_factor"] - xs["f64"]))
+                                                             / normB),
+                         "rel_x_difference": float(np.linalg.norm(xs["c64_factor"] - xs["f64"])
+                                                   / np.linalg.norm(xs["f64"]))}
+    out["value"] = out["f64"]["solves_per_s"]
+    print(json.dumps(out), flush=True)
+    m.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,7 +338,7 @@ def main():
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving"],
+    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving", "eddy"],
                     help="dropin: the n=95 solve (default); batched: 256 systems n=8..40 plus one n=60 system with "
                          "32 right-hand sides; serving: Poisson arrivals of mixed sizes / types")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -307,6 +357,8 @@ def main():
         return workload_batched(a)
     if a.workload == "serving":
         return workload_serving(a)
+    if a.workload == "eddy":
+        return workload_eddy(a)
     t0 = time.perf_counter()
     s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
     t_gen = time.perf_counter() - t0

@@ -102,6 +102,25 @@ int ssgpu_context_resident_bytes(ssgpu_context ctx, int64_t* bytes);
 int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr,
                         const int32_t* col_idx, const void* values, ssgpu_matrix* out);
 /* Replace the values (same pattern) from host memory. */
+/* Same matrix in the algebraic block multi-color (ABMC) order: rows
+ * aggregated into blocks of at most block_size, blocks colored so that a
+ * color's blocks share no entry (Iwashita et al., IPDPS 2012; the reference
+ * names this only as a planned solver, pybind/MatSolvers_pybnd_add_memo.txt:13).
+ * The library holds P A P^T and every solve on the handle gathers b and x
+ * into that order and returns x in the caller's order; the kernel-level
+ * entry points below work in the library's order.  Not a parity mode: the
+ * IC(0) factor of P A P^T is a different preconditioner (other iteration
+ * counts), in exchange for far fewer dependent sweep levels (at most
+ * colors x block_size).  *ncolors (may be NULL) receives the color count. */
+/* The ABMC order ssgpu_matrix_create_ordered uses: perm[new] = old row
+ * (host only; needs no device).  No counterpart in the reference. */
+int ssgpu_abmc_order(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, int block_size,
+                     int32_t* perm, int32_t* ncolors);
+
+int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr,
+                                const int32_t* col_idx, const void* values, int block_size,
+                                ssgpu_matrix* out, int32_t* ncolors);
+
 int ssgpu_matrix_update_values(ssgpu_matrix m, const void* values);
 int ssgpu_matrix_destroy(ssgpu_matrix m);
 int ssgpu_matrix_info(ssgpu_matrix m, int32_t* n, int64_t* nnz, int* dtype,

@@ -37,7 +37,7 @@ def _run(cmd):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_capi.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_order.hip", "ssgpu_capi.hip")]
     out = os.path.join(LIB, "libsparsesolv_amd.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
@@ -59,6 +59,7 @@ def build_cpp():
     out = os.path.join(LIB, "libSparseSolv.so")
     tmp = out + ".tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", f"-I{INC}", os.path.join(CSRC, "sparsesolv_cpp.cpp"),
+          os.path.join(CSRC, "sprs_trans.cpp"),
           f"-L{LIB}", "-lsparsesolv_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, out)
     return out

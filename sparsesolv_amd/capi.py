@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+from typing import Optional
 
 import numpy as np
 
@@ -68,6 +69,8 @@ def lib():
         L.ssgpu_context_synchronize.argtypes = [V]
         L.ssgpu_context_resident_bytes.argtypes = [V, P(ctypes.c_int64)]
         L.ssgpu_matrix_create.argtypes = [V, I, ctypes.c_int32, V, V, V, P(V)]
+        L.ssgpu_matrix_create_ordered.argtypes = [V, I, ctypes.c_int32, V, V, V, I, P(V), P(ctypes.c_int32)]
+        L.ssgpu_abmc_order.argtypes = [ctypes.c_int32, V, V, I, V, P(ctypes.c_int32)]
         L.ssgpu_matrix_update_values.argtypes = [V, V]
         L.ssgpu_matrix_destroy.argtypes = [V]
         L.ssgpu_matrix_info.argtypes = [V, P(ctypes.c_int32), P(ctypes.c_int64), P(I), P(ctypes.c_int32),
@@ -136,10 +139,24 @@ def _dtype_code(values):
     return C128 if np.iscomplexobj(values) else R64
 
 
+def abmc_order(row_ptr, col_idx, block_size):
+    """(perm, ncolors): the algebraic block multi-color order, perm[new] = old."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    perm = np.empty(len(rp) - 1, np.int32)
+    nc = ctypes.c_int32()
+    check(lib().ssgpu_abmc_order(len(perm), rp.ctypes.data, ci.ctypes.data, int(block_size), perm.ctypes.data,
+                                 ctypes.byref(nc)))
+    return perm, nc.value
+
+
 class Matrix:
     """Device-resident CSR matrix with its structural analysis."""
 
-    def __init__(self, ctx: Context, row_ptr, col_idx, values):
+    def __init__(self, ctx: Context, row_ptr, col_idx, values, abmc_block: Optional[int] = None):
+        """abmc_block: hold the matrix in algebraic block multi-color order
+        with blocks of that many rows (solves still take and return vectors
+        in the caller's order; not a parity mode)."""
         self.ctx = ctx
         self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
         self.col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
@@ -147,8 +164,16 @@ class Matrix:
         vals = np.ascontiguousarray(values, dtype=self.dtype)
         self.n = len(self.row_ptr) - 1
         self.h = ctypes.c_void_p()
-        check(lib().ssgpu_matrix_create(ctx.h, _dtype_code(vals), self.n, self.row_ptr.ctypes.data,
-                                        self.col_idx.ctypes.data, vals.ctypes.data, ctypes.byref(self.h)))
+        self.ncolors = None
+        if abmc_block is None:
+            check(lib().ssgpu_matrix_create(ctx.h, _dtype_code(vals), self.n, self.row_ptr.ctypes.data,
+                                            self.col_idx.ctypes.data, vals.ctypes.data, ctypes.byref(self.h)))
+        else:
+            nc = ctypes.c_int32()
+            check(lib().ssgpu_matrix_create_ordered(ctx.h, _dtype_code(vals), self.n, self.row_ptr.ctypes.data,
+                                                    self.col_idx.ctypes.data, vals.ctypes.data, int(abmc_block),
+                                                    ctypes.byref(self.h), ctypes.byref(nc)))
+            self.ncolors = nc.value
 
     def close(self):
         if self.h:

@@ -26,6 +26,8 @@
 
 namespace ssgpu {
 
+int abmc_order(int n, const int* rp, const int* ci, int bs, std::vector<int>& perm);
+
 thread_local std::string g_last_error;
 void set_last_error(const std::string& s) { g_last_error = s; }
 
@@ -154,6 +156,12 @@ struct Matrix {
     // (prep_precision: FP64 only, or FP64 + FP32 copies), 2 SGS values of S A S
     int prep_kind = 0;
     int prep_precision = 0;
+    // ABMC-reordered matrix: the library holds P A P^T; solves gather b and x
+    // through perm (new -> old) and return x through iperm (old -> new)
+    bool ordered = false;
+    int ncolors = 0;
+    DevBuf perm, iperm;
+    std::vector<int64_t> vmap;   // stored entry q of P A P^T = caller's entry vmap[q]
 };
 
 // levels of the DAG "row i depends on the entries of its row with index on
@@ -1425,6 +1433,37 @@ template <typename T> void factorize(Matrix& m, int solver, double accel, int pr
     SSGPU_HIP(hipStreamSynchronize(s));
 }
 
+// Solves on an ABMC-reordered matrix: b and x (ncol vectors of n) are
+// gathered into the matrix's order, solved there, and x gathered back.
+struct PermutedIO {
+    Context* c;
+    Matrix& m;
+    DevBuf bp, xp;
+    const void* b;
+    void* x;
+    int ncol;
+    PermutedIO(Context* cc, Matrix& mm, const void* bb, void* xx, int nc) : c(cc), m(mm), b(bb), x(xx), ncol(nc) {
+        const size_t bytes = m.vsize * (size_t)m.n * nc;
+        bp.alloc(c, bytes);
+        xp.alloc(c, bytes);
+        for (int j = 0; j < nc; ++j) {
+            gather(static_cast<const char*>(b) + m.vsize * (size_t)m.n * j, static_cast<char*>(bp.p) + m.vsize * (size_t)m.n * j, m.perm);
+            gather(static_cast<const char*>(x) + m.vsize * (size_t)m.n * j, static_cast<char*>(xp.p) + m.vsize * (size_t)m.n * j, m.perm);
+        }
+    }
+    void gather(const void* src, void* dst, const DevBuf& map) {
+        if (m.dtype == SSGPU_R64)
+            launch_gather<double, double>(c->stream, m.n, map.as<int>(), (const double*)src, (double*)dst);
+        else
+            launch_gather<zc, zc>(c->stream, m.n, map.as<int>(), (const zc*)src, (zc*)dst);
+    }
+    void finish() {   // x <- xp in the caller's order; waits for the copies
+        for (int j = 0; j < ncol; ++j)
+            gather(static_cast<const char*>(xp.p) + m.vsize * (size_t)m.n * j, static_cast<char*>(x) + m.vsize * (size_t)m.n * j, m.iperm);
+        SSGPU_HIP(hipStreamSynchronize(c->stream));
+    }
+};
+
 }  // namespace ssgpu
 
 // ==================================================================== C-ABI
@@ -1587,11 +1626,81 @@ int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* 
     SSGPU_GUARD_END
 }
 
+int ssgpu_abmc_order(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, int block_size, int32_t* perm,
+                     int32_t* ncolors) {
+    if (n <= 0 || !row_ptr || !col_idx || !perm || block_size < 1) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    std::vector<int> p;
+    const int nc = abmc_order(n, row_ptr, col_idx, block_size, p);
+    std::copy(p.begin(), p.end(), perm);
+    if (ncolors) *ncolors = nc;
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr,
+                                const int32_t* col_idx, const void* values, int block_size, ssgpu_matrix* out,
+                                int32_t* ncolors) {
+    if (!ctx || !out || !row_ptr || !col_idx || !values || n <= 0 || block_size < 1 ||
+        (dtype != SSGPU_R64 && dtype != SSGPU_C128))
+        return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    for (int i = 0; i < n; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) throw Error{BAD_ARG, "row_ptr not monotone"};
+    const int64_t nnz = row_ptr[n];
+    for (int64_t k = 0; k < nnz; ++k)
+        if (col_idx[k] < 0 || col_idx[k] >= n) throw Error{NOT_SQUARE, "column index outside the square matrix"};
+    std::vector<int> perm;
+    const int nc = abmc_order(n, row_ptr, col_idx, block_size, perm);
+    std::vector<int> iperm(n);
+    for (int r = 0; r < n; ++r) iperm[perm[r]] = r;
+    // P A P^T with ascending columns per row
+    const size_t vs = dtype == SSGPU_R64 ? 8 : 16;
+    std::vector<int> prp(n + 1, 0), pci((size_t)nnz);
+    std::vector<char> pv(vs * (size_t)nnz);
+    std::vector<int64_t> vmap((size_t)nnz);
+    std::vector<std::pair<int, int64_t>> row;
+    for (int r = 0; r < n; ++r) {
+        const int i = perm[r];
+        row.clear();
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) row.emplace_back(iperm[col_idx[k]], k);
+        std::sort(row.begin(), row.end());
+        int q = prp[r];
+        for (const auto& e : row) {
+            pci[q] = e.first;
+            vmap[q] = e.second;
+            std::memcpy(&pv[vs * q], static_cast<const char*>(values) + vs * e.second, vs);
+            ++q;
+        }
+        prp[r + 1] = q;
+    }
+    ssgpu_matrix h = nullptr;
+    const int st = ssgpu_matrix_create(ctx, dtype, n, prp.data(), pci.data(), pv.data(), &h);
+    if (st != SSGPU_OK) return st;
+    Matrix& m = h->m;
+    m.ordered = true;
+    m.ncolors = nc;
+    m.vmap = std::move(vmap);
+    upload(m.ctx, m.perm, perm);
+    upload(m.ctx, m.iperm, iperm);
+    if (ncolors) *ncolors = nc;
+    *out = h;
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
 int ssgpu_matrix_update_values(ssgpu_matrix mh, const void* values) {
     if (!mh || (!values && mh->m.nnz)) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
     Matrix& m = mh->m;
     SSGPU_HIP(hipSetDevice(m.ctx->device));
+    std::vector<char> pv;
+    if (m.ordered) {   // caller's entry order -> the stored P A P^T order
+        pv.resize(m.vsize * (size_t)m.nnz);
+        for (int64_t q = 0; q < m.nnz; ++q)
+            std::memcpy(&pv[m.vsize * q], static_cast<const char*>(values) + m.vsize * m.vmap[q], m.vsize);
+        values = pv.data();
+    }
     if (m.nnz) SSGPU_HIP(hipMemcpyAsync(m.val.p, values, m.vsize * m.nnz, hipMemcpyHostToDevice, m.ctx->stream));
     SSGPU_HIP(hipStreamSynchronize(m.ctx->stream));
     m.factor_valid = false;
@@ -1644,10 +1753,19 @@ int ssgpu_solve_device(ssgpu_matrix mh, int solver, const ssgpu_options* opt, do
     Matrix& m = mh->m;
     std::lock_guard<std::mutex> lk(m.ctx->mu);
     SSGPU_HIP(hipSetDevice(m.ctx->device));
-    if (m.dtype == SSGPU_R64)
-        return solve_device<double>(m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b, (double*)d_x, log,
-                                    log_cap, res);
-    return solve_device<zc>(m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log, log_cap, res);
+    std::unique_ptr<PermutedIO> pio;
+    if (m.ordered) {
+        pio.reset(new PermutedIO(m.ctx, m, d_b, d_x, 1));
+        d_b = pio->bp.p;
+        d_x = pio->xp.p;
+    }
+    const int st = m.dtype == SSGPU_R64
+                       ? solve_device<double>(m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b,
+                                              (double*)d_x, log, log_cap, res)
+                       : solve_device<zc>(m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log,
+                                          log_cap, res);
+    if (pio) pio->finish();
+    return st;
     SSGPU_GUARD_END
 }
 
@@ -1678,11 +1796,19 @@ int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix mh, int solver, const s
     if (c->device != m.ctx->device) throw Error{BAD_ARG, "context and matrix are on different devices"};
     std::lock_guard<std::mutex> lk(c->mu);
     SSGPU_HIP(hipSetDevice(c->device));
-    if (m.dtype == SSGPU_R64)
-        return solve_factored<double>(c, m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b,
-                                      (double*)d_x, log, log_cap, res);
-    return solve_factored<zc>(c, m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log, log_cap,
-                              res);
+    std::unique_ptr<PermutedIO> pio;
+    if (m.ordered) {
+        pio.reset(new PermutedIO(c, m, d_b, d_x, 1));
+        d_b = pio->bp.p;
+        d_x = pio->xp.p;
+    }
+    const int st = m.dtype == SSGPU_R64
+                       ? solve_factored<double>(c, m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b,
+                                                (double*)d_x, log, log_cap, res)
+                       : solve_factored<zc>(c, m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x,
+                                            log, log_cap, res);
+    if (pio) pio->finish();
+    return st;
     SSGPU_GUARD_END
 }
 
@@ -1697,9 +1823,18 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix mh, const ssgpu_options* o
     if (c->device != m.ctx->device) throw Error{BAD_ARG, "context and matrix are on different devices"};
     std::lock_guard<std::mutex> lk(c->mu);
     SSGPU_HIP(hipSetDevice(c->device));
-    if (m.dtype == SSGPU_R64)
-        return solve_block<double>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const double*)d_B, (double*)d_X, res);
-    return solve_block<zc>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const zc*)d_B, (zc*)d_X, res);
+    std::unique_ptr<PermutedIO> pio;
+    if (m.ordered) {
+        pio.reset(new PermutedIO(c, m, d_B, d_X, nrhs));
+        d_B = pio->bp.p;
+        d_X = pio->xp.p;
+    }
+    const int st = m.dtype == SSGPU_R64
+                       ? solve_block<double>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const double*)d_B,
+                                             (double*)d_X, res)
+                       : solve_block<zc>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const zc*)d_B, (zc*)d_X, res);
+    if (pio) pio->finish();
+    return st;
     SSGPU_GUARD_END
 }
 
