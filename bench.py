@@ -32,6 +32,8 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK_GBS = 8000.0
 SOLVERS = {"iccg": 0, "mrtr_ic": 1, "mrtr_sgs": 2}
+# rows per block of the ABMC (non-parity) ordering the bench reports beside the drop-in
+ABMC_BLOCK = 1
 
 
 def algorithmic_bytes(n, nnzA, nnzL, s_a=8, s_f=8, s_v=8):
@@ -321,6 +323,101 @@ def workload_eddy(a):
     ctx.close()
 
 
+def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
+    """The same solve on the algebraic block multi-color reordering of the
+    system (not a parity mode: a different IC(0) preconditioner, so other
+    iteration counts; x is returned in the caller's order).  Reported beside
+    the drop-in with its own sweep roofline."""
+    import torch
+    from sparsesolv_amd import capi
+    t0 = time.perf_counter()
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=ABMC_BLOCK)
+    t_an = time.perf_counter() - t0
+    info = m.info()
+    d_x = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    m.solve_device(code, d_b.data_ptr(), d_x.data_ptr(), conv_cri=a.tol, max_ite=20000, normB=normB, opts=opts)
+    lat = []
+    for _ in range(max(1, a.steps)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = m.solve_device(code, d_b.data_ptr(), d_x.data_ptr(), conv_cri=a.tol, max_ite=20000, normB=normB,
+                           opts=opts)
+        lat.append(time.perf_counter() - t)
+    x = d_x.cpu().numpy()
+    true_res = float(np.linalg.norm(s.rhs - s.scipy() @ x) / normB)
+    d_y = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    t_spmv = m.time_kernel(0, a.reps, d_b.data_ptr(), d_y.data_ptr())
+    t_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
+    t_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
+    m.close()
+    nnzL = (s.nnz + s.n) // 2
+    b_spmv, b_sweep = algorithmic_bytes(s.n, s.nnz, nnzL, s_f=4 if prec == capi.FACTOR_F32 else 8)
+    sweep_ms = 0.5 * (t_fwd + t_bwd)
+    gbs = b_sweep / (sweep_ms * 1e-3) / 1e9
+    med = statistics.median(lat)
+    print(f"# abmc (block {ABMC_BLOCK}): {1e3 * med:.1f} ms/solve, {r.iterations} iterations, "
+          f"{info['fwd_levels']} levels, sweep {sweep_ms:.3f} ms ({gbs:.0f} GB/s)", file=sys.stderr, flush=True)
+    return {"mode": "not parity: IC(0) of the ABMC-reordered system", "block_rows": ABMC_BLOCK,
+            "colors": m.ncolors, "levels": info["fwd_levels"], "analysis_s": t_an,
+            "solves_per_s": 1.0 / med, "ms_per_solve": 1e3 * med, "iterations": r.iterations,
+            "converged": bool(r.converged), "true_residual": true_res,
+            "roofline_sweep": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
+            "spmv_ms": t_spmv}
+
+
+def workload_voxelfem(a):
+    """The reference example's own problem (examples/VoxelFEM): the 95^3
+    graded mesh (0.001 m cells up to 0.05 m, 0.01 m beyond), iron and coil by
+    its rules, n x A = 0 on x = 0, y = 0 and the outer faces, T-method source;
+    assembled from the mesh description through the .vxldata path.  Shifted
+    ICCG at the example's acceleration 1.1 to 1e-8: natural order (parity
+    mode) and ABMC order (non-parity)."""
+    import torch
+    from sparsesolv_amd import capi, voxel
+    t0 = time.perf_counter()
+    s = voxel.voxelfem_system()
+    print(f"# voxelfem mesh system N={s.n} nnz={s.nnz} ({time.perf_counter() - t0:.1f}s)", file=sys.stderr,
+          flush=True)
+    ctx = capi.Context(0)
+    normB = float(np.linalg.norm(s.rhs))
+    d_b = torch.from_numpy(s.rhs).cuda()
+    out = {"metric": "shifted ICCG solves/s on the reference example's problem", "unit": "solves/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64",
+           "data": "the example's graded 95^3 voxel mesh, materials, boundary and coil source (examples/VoxelFEM)",
+           "config": {"workload": f"VoxelFEM magnetostatic, N={s.n}, nnz={s.nnz}, shift 1.1, tol 1e-8"}}
+    A = s.scipy()
+    for name, abmc in (("natural", None), ("abmc", ABMC_BLOCK)):
+        m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=abmc)
+        opts = capi.default_options(accel_factor=1.1)
+        d_x = torch.zeros_like(d_b)
+        torch.cuda.synchronize()
+        r = m.solve_device(capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=20000, normB=normB,
+                           opts=opts)
+        lat = []
+        for _ in range(max(1, a.steps)):
+            d_x.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = m.solve_device(capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=20000,
+                               normB=normB, opts=opts)
+            lat.append(time.perf_counter() - t)
+        x = d_x.cpu().numpy()
+        info = m.info()
+        out[name] = {"solves_per_s": 1.0 / statistics.median(lat), "ms_per_solve": 1e3 * statistics.median(lat),
+                     "iterations": r.iterations, "converged": bool(r.converged),
+                     "true_residual": float(np.linalg.norm(s.rhs - A @ x) / normB),
+                     "levels": info["fwd_levels"], "colors": m.ncolors}
+        print(f"# voxelfem {name}: {out[name]['ms_per_solve']:.1f} ms, {r.iterations} iterations",
+              file=sys.stderr, flush=True)
+        m.close()
+    out["value"] = out["natural"]["solves_per_s"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,12 +430,13 @@ def main():
     ap.add_argument("--ordering", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--abmc", type=int, default=1, help="also time the ABMC-ordered (non-parity) solve (0: skip)")
     ap.add_argument("--reps", type=int, default=10, help="repetitions for the per-kernel timing")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving", "eddy"],
+    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving", "eddy", "voxelfem"],
                     help="dropin: the n=95 solve (default); batched: 256 systems n=8..40 plus one n=60 system with "
                          "32 right-hand sides; serving: Poisson arrivals of mixed sizes / types")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -359,6 +457,8 @@ def main():
         return workload_serving(a)
     if a.workload == "eddy":
         return workload_eddy(a)
+    if a.workload == "voxelfem":
+        return workload_voxelfem(a)
     t0 = time.perf_counter()
     s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
     t_gen = time.perf_counter() - t0
@@ -422,6 +522,8 @@ def main():
     print(f"# per-iteration {per_it_ms:.3f} ms; spmv {t_spmv:.3f} ms ({gbs_spmv:.0f} GB/s); "
           f"fwd {t_fwd:.3f} ms bwd {t_bwd:.3f} ms (sweep {gbs_sweep:.0f} GB/s)", file=sys.stderr, flush=True)
 
+    abmc = abmc_run(a, s, ctx, d_b, normB, code, prec, opts) if a.abmc else None
+
     served = None
     if a.serve_slots > 0 and a.solver != "mrtr_sgs":
         served = served_run(a, s, code, m.close)
@@ -467,6 +569,7 @@ def main():
                           "algorithmic_bytes": b_spmv, "avg_ms": t_spmv},
         "cpu_baseline": cpu,
         "served": served,
+        "abmc": abmc,
     }
     if served:
         served["vs_plain_fp64"] = {k: v / out["value"] for k, v in served["solves_per_s"].items()}

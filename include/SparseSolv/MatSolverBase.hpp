@@ -37,6 +37,7 @@ class MatSolverBase {
     int device = 0;
     int factor_precision = 0;     /* 0 FP64 factor, 1 FP32 factor (FP64 pivots / recurrences) */
     bool strict_exit = false;     /* missing diagonal: false throws, true prints and exit(1) like the reference */
+    int abmc_block = 0;
     int last_iterations = 0;
     double last_setup_ms = 0.0, last_iterate_ms = 0.0;
 
@@ -61,6 +62,10 @@ class MatSolverBase {
     void setDevice(int d) { device = d; }
     void setFactorPrecision(int p) { factor_precision = p; }
     void setStrictExit(bool b) { strict_exit = b; }
+    /* 0: the matrix's own row order (parity with the reference); b > 0: solve
+       on the algebraic block multi-color reordering with blocks of b rows
+       (fewer dependent sweep levels, different iteration counts) */
+    void setABMC(int block_size) { abmc_block = block_size > 0 ? block_size : 0; }
     int getIterations() const { return last_iterations; }
     double getSetupMs() const { return last_setup_ms; }
     double getIterateMs() const { return last_iterate_ms; }

@@ -85,6 +85,34 @@ int ssv_unknown_edges(const ssv_params* p, int32_t* dir, int32_t* i, int32_t* j,
  * (i,j,k) -> unknown index or -1 when the edge is a Dirichlet edge. */
 int32_t ssv_edge_unknown(const ssv_params* p, int i, int j, int k, int l);
 
+/* Element matrices (as ssv_element_matrices) of an axis-aligned box with
+ * edge lengths hx, hy, hz: the example's element matrix takes three half
+ * lengths (examples/VoxelFEM/VoxelMatrix.cpp:133-136). */
+void ssv_box_element_matrices(double hx, double hy, double hz, double* K, double* M, double* C);
+
+/* ---- mesh-driven assembly (the reference example's mesh files) ----------
+ * A hexahedral mesh as examples/VoxelFEM/VoxelMesh.cpp:38-97 reads it:
+ *   xyz            double[3*n_nodes]      node coordinates
+ *   edge_nodes     int32[2*n_edges]       start and end node of each edge
+ *   element_edges  int32[12*n_elements]   the 12 edges of each element
+ * Every element must be an axis-aligned box (edges in any order and either
+ * orientation); an edge's unknown is the line integral from its start node
+ * to its end node.  zero_edge (uint8[n_edges], NULL = none) marks n x A = 0
+ * edges; the other edges are numbered in file order (VoxelMesh.cpp:101-122).
+ * nu and sigma (NULL = 0) are per element; t_edge (NULL = no source) holds
+ * the line integral of the current potential T along each edge, start to
+ * end (setT_Vector.cpp:27-57); b_k = int N_k . curl T.
+ * Returns 0, or 1 for bad arguments, 2 for more than 2^31-1 entries, 3 for
+ * an element that is not an axis-aligned box or an index out of range. */
+int ssv_mesh_count(int32_t n_edges, const int32_t* edge_nodes, int32_t n_elements,
+                   const int32_t* element_edges, const uint8_t* zero_edge,
+                   int32_t* n_unknowns, int64_t* nnz);
+int ssv_mesh_build(int32_t n_nodes, const double* xyz, int32_t n_edges, const int32_t* edge_nodes,
+                   int32_t n_elements, const int32_t* element_edges, const double* nu,
+                   const double* sigma, double omega, const double* t_edge,
+                   const uint8_t* zero_edge, int complex_values,
+                   int32_t* row_ptr, int32_t* col_idx, double* values, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

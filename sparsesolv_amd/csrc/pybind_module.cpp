@@ -95,6 +95,7 @@ PYBIND11_MODULE(SparseSolvPy, m) {
         // additions of the GPU library
         .def("setDevice", &MatSolverBase::setDevice)
         .def("setFactorPrecision", &MatSolverBase::setFactorPrecision)
+        .def("setABMC", &MatSolverBase::setABMC, py::arg("block_size"))
         .def("getIterations", &MatSolverBase::getIterations)
         .def("getSetupMs", &MatSolverBase::getSetupMs)
         .def("getIterateMs", &MatSolverBase::getIterateMs);

@@ -70,6 +70,7 @@ struct DeviceMatrixCache {
     ssgpu_matrix h = nullptr;
     uint64_t version = 0;
     int device = -1;
+    int abmc = 0;
     ~DeviceMatrixCache() {
         if (h) ssgpu_matrix_destroy(h);
     }
@@ -107,19 +108,23 @@ template <typename D> constexpr int dtype_code();
 template <> constexpr int dtype_code<double>() { return SSGPU_R64; }
 template <> constexpr int dtype_code<dcomplex>() { return SSGPU_C128; }
 
-template <typename D> ssgpu_matrix device_matrix(const SparseMatTMPL<D>& A, int device) {
+template <typename D> ssgpu_matrix device_matrix(const SparseMatTMPL<D>& A, int device, int abmc) {
     auto& c = A.deviceCache();
-    if (c && c->h && c->version == A.version() && c->device == device) return c->h;
+    if (c && c->h && c->version == A.version() && c->device == device && c->abmc == abmc) return c->h;
     if (A.getSize() != A.getColSize())
         throw std::runtime_error("SparseSolv: the solvers need a square matrix");
     auto nc = std::make_shared<DeviceMatrixCache>();
-    const int st = ssgpu_matrix_create(context_for(device), dtype_code<D>(), A.getSize(), A.getRowPtr(),
-                                       A.getColPtr(), (const void*)A.getValuePtr(), &nc->h);
+    const int st = abmc > 0 ? ssgpu_matrix_create_ordered(context_for(device), dtype_code<D>(), A.getSize(),
+                                                          A.getRowPtr(), A.getColPtr(), (const void*)A.getValuePtr(),
+                                                          abmc, &nc->h, nullptr)
+                            : ssgpu_matrix_create(context_for(device), dtype_code<D>(), A.getSize(), A.getRowPtr(),
+                                                  A.getColPtr(), (const void*)A.getValuePtr(), &nc->h);
     if (st != SSGPU_OK)
         throw std::runtime_error(std::string("SparseSolv: matrix upload failed: ") + ssgpu_status_string(st) + ": " +
                                  ssgpu_last_error());
     nc->version = A.version();
     nc->device = device;
+    nc->abmc = abmc;
     c = nc;
     return c->h;
 }
@@ -441,7 +446,7 @@ void MatSolverBase::getResidualLog(std::vector<double>& log) {
 
 bool MatSolverBase::run(int kind, double accel, int mrtr_fix, const double conv_cri, const int max_ite,
                         const double normB, const SparseMat* A, const SparseMatC* Ac, const void* b, void* x) {
-    ssgpu_matrix h = A ? device_matrix(*A, device) : device_matrix(*Ac, device);
+    ssgpu_matrix h = A ? device_matrix(*A, device, abmc_block) : device_matrix(*Ac, device, abmc_block);
     ssgpu_options o;
     ssgpu_options_default(&o);
     o.is_init = is_init;

@@ -118,6 +118,8 @@ struct SfCsr {
     int kmax = 0;          // longest row; 0 when a row exceeds 64 entries (level path used)
     int64_t nell = 0;
     const int* srow = nullptr;   // level position -> row (the LevelCsr's prow)
+    const LevelCsr* lv = nullptr;   // its levels (host level starts)
+    bool by_level = false;       // one launch per level instead of the sync-free sweep
     DevBuf slice_off, rlen, ecol, emap_f, dmap_f, emap_a, dmap_a;
     DevBuf eval, dval, eval32, dval32;
 };
@@ -143,6 +145,7 @@ struct Matrix {
     int ic_max_updates = 0;    // largest update list of a row
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
     int sweep_grid = 128;     // persistent one-wave workgroups per sweep
+    bool sweeps_tuned = false;   // sweep launch shape measured (first factorization)
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
     int block_kc = 0;         // block sweeps: columns per lane (0: 1)
     int block_grid = 0;       // block sweeps: workgroups per column group (0: about 512 waves in all)
@@ -228,6 +231,11 @@ static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const
 // Per original row: its off-diagonal entries (col, source in the factor,
 // source in A) in summation order, and its diagonal's sources; flat arrays
 // with row offsets (entries of row i at off[i] .. off[i+1]).
+// mean rows per level from which a sweep runs one launch per level
+constexpr int64_t kLevelSyncRows = 131072;
+// most levels for which the one-launch-per-level sweep is timed by the tuner
+constexpr int kTuneMaxLevels = 512;
+
 struct SfRows {
     std::vector<int> off, col, sf, sa, df, da;
     explicit SfRows(int n) : off(n + 1, 0), df(n, -1), da(n, -1) {}
@@ -270,6 +278,15 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
     out.kmax = kmax <= 64 ? std::max(kmax, 1) : 0;
     out.nell = nell;
     out.srow = lv.prow.as<int>();
+    out.lv = &lv;
+    // Wide levels (a multi-color order: a few levels of many rows) run one
+    // launch per level: plain loads, no readiness polling.  Narrow levels (the
+    // natural order: ~10^3 levels of ~10^3 rows) stay sync-free.
+    out.by_level = lv.nlev > 0 && (int64_t)n >= kLevelSyncRows * (int64_t)lv.nlev;
+    if (const char* e = std::getenv("SSGPU_SWEEP_MODE")) {
+        if (!std::strcmp(e, "level")) out.by_level = true;
+        if (!std::strcmp(e, "sf")) out.by_level = false;
+    }
     upload(c, out.slice_off, soff);
     upload(c, out.rlen, rlen);
     upload(c, out.ecol, ecol);
@@ -413,6 +430,10 @@ static void analyse(Matrix& m) {
         build_sf(c, n, order_l, rl, m.L, m.sfL);
         build_sf(c, n, order_u, ru, m.Uic, m.sfUic);
     }
+    // sync-free grid: 128 waves keep ~10^3-row levels fed with the least
+    // polling; levels of thousands of rows want more waves in flight
+    if (!std::getenv("SSGPU_SWEEP_GRID") && m.L.nlev > 0 && (int64_t)n >= 2048 * (int64_t)m.L.nlev)
+        m.sweep_grid = 1024;
     // upper(A) for the SGS sweeps: built on first use (ensure_upper)
 }
 
@@ -506,8 +527,14 @@ struct Solver {
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
                   const T* aux, const T* dd, T* x) {
         (void)reverse;   // backward structures are already in their own level order
-        launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), S.srow, S.slice_off.as<int>(),
-                              S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd, m.sweep_idle, x);
+        if (S.by_level)
+            launch_sweep_levels<T, F>(s, mode, S.kmax, S.lv->nlev, S.lv->levptr.data(), S.srow,
+                                      S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux,
+                                      dd, x);
+        else
+            launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), S.srow,
+                                  S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd,
+                                  m.sweep_idle, x);
     }
     // values of a sliced-ELL copy from a source array (factor or matrix values)
     template <typename T> void sf_values(SfCsr& S, const T* src, bool from_factor, bool with_f32) {
@@ -653,6 +680,55 @@ struct Solver {
         return true;
     }
     // Factor the matrix with values av (A or the scaled copy); returns pivot fixes.
+    // Sweep launch shape, measured once per matrix on its first factor: the
+    // forward sweep is timed sync-free with 128 and with 1024 waves, and one
+    // launch per level when the levels are few; the fastest is kept for the
+    // IC sweeps of this matrix.  Every shape gives the same bits.
+    template <typename T> void tune_sweeps() {
+        m.sweeps_tuned = true;
+        const bool fix_mode = std::getenv("SSGPU_SWEEP_MODE") != nullptr;
+        const bool fix_grid = std::getenv("SSGPU_SWEEP_GRID") != nullptr;
+        if (fix_mode && fix_grid) return;
+        DevBuf r, out;
+        r.alloc(c, sizeof(T) * m.n);
+        out.alloc(c, sizeof(T) * m.n);
+        launch_fill<T>(s, m.n, r.as<T>(), one<T>());
+        struct Shape { bool level; int grid; };
+        std::vector<Shape> cand;
+        if (!fix_mode || !m.sfL.by_level) {
+            for (int g : {128, 1024})
+                if (!fix_grid || g == m.sweep_grid) cand.push_back({false, fix_grid ? m.sweep_grid : g});
+        }
+        if ((!fix_mode || m.sfL.by_level) && m.L.nlev <= kTuneMaxLevels) cand.push_back({true, m.sweep_grid});
+        if (cand.size() < 2) return;
+        hipEvent_t e0, e1;
+        SSGPU_HIP(hipEventCreate(&e0));
+        SSGPU_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        Shape pick = cand[0];
+        for (const Shape& sh : cand) {
+            m.sfL.by_level = sh.level;
+            m.sweep_grid = sh.grid;
+            float t = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
+                launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
+                SSGPU_HIP(hipEventRecord(e0, s));
+                sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(),
+                               r.as<T>(), nullptr, nullptr, out.as<T>());
+                SSGPU_HIP(hipEventRecord(e1, s));
+                SSGPU_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                SSGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) t = std::min(t, ms);
+            }
+            if (t < best) { best = t; pick = sh; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        check_sweep_error();
+        m.sfL.by_level = m.sfUic.by_level = pick.level;
+        m.sweep_grid = pick.grid;
+    }
     template <typename T> int factor(const T* av, double accel) {
         const int64_t nl = m.L.nnz;
         if (m.Lv.bytes < sizeof(T) * nl) m.Lv.alloc(c, sizeof(T) * nl);
@@ -687,6 +763,7 @@ struct Solver {
             const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
             sf_values<T>(m.sfL, m.Lv.as<T>(), true, f32);
             sf_values<T>(m.sfUic, m.Lv.as<T>(), true, f32);
+            if (!m.sweeps_tuned) tune_sweeps<T>();
         }
         if (o.factor_precision == SSGPU_FACTOR_F32) {
             using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;

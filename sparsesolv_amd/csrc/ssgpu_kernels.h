@@ -133,6 +133,12 @@ template <typename T, typename F>
 void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, int max_idle, T* x);
+// the same sweep one level per launch (levptr: host array of nlev + 1 level
+// starts in the level order); for orderings with few, wide levels
+template <typename T, typename F>
+void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int* levptr, const int* srow,
+                         const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
+                         const T* rhs, const T* aux, const T* dd, T* x);
 // block sweep over ncol right-hand sides (column c at offset c * n), kc in
 // {1, 2, 4 (real only)} columns per lane (registers: KMAX x kc inputs),
 // ncol a multiple of kc; tickets: ncol / kc zeroed counters

@@ -33,6 +33,7 @@ namespace ssgpu {
 namespace {
 
 constexpr unsigned long long kSentinel = 0xFFF40000DEADBEEFull;
+constexpr int kLvlWaves = 4;   // waves (64-row chunks) per workgroup of the per-level sweep
 constexpr long kSpinLimit = 1L << 23;
 
 __device__ inline bool sent(double v) { return (unsigned long long)__double_as_longlong(v) == kSentinel; }
@@ -73,7 +74,8 @@ template <typename T, typename F, int MODE, int KMAX> struct RowState {
     F vals[KMAX];
     T r_i, a_i, d_i;
     F dg;
-    __device__ void load(int p, bool active, int c, const int* __restrict__ srow, const int* __restrict__ slice_off,
+    __device__ void load(int p, bool active, int c, int lane, const int* __restrict__ srow,
+                         const int* __restrict__ slice_off,
                          const int* __restrict__ rlen, const int* __restrict__ ecol, const F* __restrict__ eval,
                          const F* __restrict__ dval, const T* __restrict__ rhs, const T* __restrict__ aux,
                          const T* __restrict__ dd) {
@@ -81,7 +83,7 @@ template <typename T, typename F, int MODE, int KMAX> struct RowState {
         dg = F();
         i = active ? srow[p] : 0;
         len = active ? rlen[p] : 0;
-        const int base = slice_off[c] + (int)threadIdx.x;
+        const int base = active ? slice_off[c] + lane : 0;   // inactive lanes read nothing
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
             if (k < len) {
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
         const int p = c * kSfRows + lane;   // position in level order
         const bool active = p < n;
         RowState<T, F, MODE, KMAX> row;
-        row.load(p, active, c, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
+        row.load(p, active, c, lane, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
         T g[KMAX];
         using Mask = typename std::conditional<(KMAX <= 32), unsigned, unsigned long long>::type;
         constexpr Mask kOne = 1;
@@ -180,6 +182,34 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
     }
 }
 
+// One level of a sweep, for orderings whose levels are wide (few levels of
+// many rows each, as a multi-color order has): every input of a row of level
+// l was written by an earlier launch on the same stream, so plain loads see
+// it and no readiness test is needed.  Positions [p0, p1) of the level order;
+// one workgroup per 64-position chunk, lanes outside the level idle.  The
+// row's result is formed by the same code, in the same order, as in the
+// sync-free sweep.
+template <typename T, typename F, int MODE, int KMAX, int WPB>
+__global__ __launch_bounds__(kSfRows * WPB) void k_sweep_lvl(int p0, int p1, const int* __restrict__ srow,
+                                                       const int* __restrict__ slice_off,
+                                                       const int* __restrict__ rlen, const int* __restrict__ ecol,
+                                                       const F* __restrict__ eval, const F* __restrict__ dval,
+                                                       const T* __restrict__ rhs, const T* __restrict__ aux,
+                                                       const T* __restrict__ dd, T* __restrict__ x) {
+    const int lane = (int)threadIdx.x % kSfRows;
+    const int c = p0 / kSfRows + (int)blockIdx.x * WPB + (int)threadIdx.x / kSfRows;
+    const int p = c * kSfRows + lane;
+    const bool active = p >= p0 && p < p1;
+    RowState<T, F, MODE, KMAX> row;
+    row.load(p, active, c, lane, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
+    if (!active) return;
+    T g[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+        if (k < row.len) g[k] = x[row.cols[k]];
+    x[row.i] = row.result(g);
+}
+
 // Several right-hand sides at once (block solves): x, rhs and aux hold ncol
 // vectors of n entries (entry (i, c) at c * n + i).  Workgroup (., g) sweeps
 // columns [g*KC, g*KC + KC) with its own ticket counter tickets[g]; a lane
@@ -206,7 +236,7 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int
         const int len = active ? rlen[p] : 0;
         int cols[KMAX];
         F vals[KMAX];
-        const int base = slice_off[c] + lane;
+        const int base = active ? slice_off[c] + lane : 0;   // inactive lanes read nothing
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
             if (k < len) {
@@ -391,6 +421,44 @@ void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ti
                                         eval, dval, rhs, aux, dd, x);
 }
 
+template <typename T, typename F, int MODE>
+void sweep_lvl_kmax(hipStream_t s, int kmax, int p0, int p1, const int* srow, const int* slice_off, const int* rlen,
+                    const int* ecol, const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+    // chunks of the level, kLvlWaves per workgroup (a chunk past the level's
+    // last one only exists inside the last workgroup: its lanes are inactive)
+    const int nch = (p1 - 1) / kSfRows - p0 / kSfRows + 1;
+    const int g = (nch + kLvlWaves - 1) / kLvlWaves;
+#define SSGPU_LV(K)                                                                                              \
+    hipLaunchKernelGGL((k_sweep_lvl<T, F, MODE, K, kLvlWaves>), dim3(g), dim3(kSfRows * kLvlWaves), 0, s, p0, p1, \
+                       srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+    if (kmax <= 8) SSGPU_LV(8);
+    else if (kmax <= 12) SSGPU_LV(12);
+    else if (kmax <= 16) SSGPU_LV(16);
+    else if (kmax <= 20) SSGPU_LV(20);
+    else if (kmax <= 24) SSGPU_LV(24);
+    else if (kmax <= 32) SSGPU_LV(32);
+    else SSGPU_LV(64);
+#undef SSGPU_LV
+}
+
+template <typename T, typename F>
+void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int* levptr, const int* srow,
+                         const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
+                         const T* rhs, const T* aux, const T* dd, T* x) {
+    for (int l = 0; l < nlev; ++l) {
+        const int p0 = levptr[l], p1 = levptr[l + 1];
+        if (p1 <= p0) continue;
+        if (mode == FWD_DIV_LAST)
+            sweep_lvl_kmax<T, F, FWD_DIV_LAST>(s, kmax, p0, p1, srow, slice_off, rlen, ecol, eval, dval, rhs, aux,
+                                               dd, x);
+        else if (mode == BWD_IC)
+            sweep_lvl_kmax<T, F, BWD_IC>(s, kmax, p0, p1, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
+        else
+            sweep_lvl_kmax<T, F, BWD_DIV_FIRST>(s, kmax, p0, p1, srow, slice_off, rlen, ecol, eval, dval, rhs, aux,
+                                                dd, x);
+    }
+}
+
 template <typename T, typename F, int KC>
 void sweep_block_kc(hipStream_t s, int mode, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
                     const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
@@ -459,6 +527,18 @@ template void launch_sweep_block<zc, zc>(hipStream_t, int, int, int, int, int, i
 template void launch_sweep_block<zc, fc>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
                                             const int*, const int*, const int*, const fc*, const fc*, const zc*,
                                             const zc*, const zc*, zc*);
+template void launch_sweep_levels<double, double>(hipStream_t, int, int, int, const int*, const int*, const int*,
+                                             const int*, const int*, const double*, const double*, const double*, const double*,
+                                             const double*, double*);
+template void launch_sweep_levels<double, float>(hipStream_t, int, int, int, const int*, const int*, const int*,
+                                             const int*, const int*, const float*, const float*, const double*, const double*,
+                                             const double*, double*);
+template void launch_sweep_levels<zc, zc>(hipStream_t, int, int, int, const int*, const int*, const int*,
+                                             const int*, const int*, const zc*, const zc*, const zc*, const zc*,
+                                             const zc*, zc*);
+template void launch_sweep_levels<zc, fc>(hipStream_t, int, int, int, const int*, const int*, const int*,
+                                             const int*, const int*, const fc*, const fc*, const zc*, const zc*,
+                                             const zc*, zc*);
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);

@@ -103,6 +103,37 @@ double inner(const VField& u, const VField& v, double h) {
     return s;
 }
 
+// The same basis on an axis-aligned box of edge lengths H[0..2]: the edge
+// function of direction d carries 1/H[d] (unit line integral along its edge)
+// and each 1-D factor is integrated over its own axis length.
+VField basis_box(int l, const double* H) {
+    VField v = basis(l, 1.0);
+    int d, a, b;
+    local_edge(l, d, a, b);
+    v.comp[d].c = 1.0 / H[d];
+    return v;
+}
+
+VField curl_box(int l, const double* H) {
+    VField v = curl_basis(l, 1.0);
+    int d, a, b;
+    local_edge(l, d, a, b);
+    for (int q = 0; q < 3; ++q)
+        if (v.comp[q].c != 0.0) v.comp[q].c = (v.comp[q].c > 0 ? 1.0 : -1.0) / H[d];
+    return v;
+}
+
+double inner_box(const VField& u, const VField& v, const double* H) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        if (u.comp[d].c == 0.0 || v.comp[d].c == 0.0) continue;
+        double p = u.comp[d].c * v.comp[d].c;
+        for (int x = 0; x < 3; ++x) p *= int1d(u.comp[d].f[x], v.comp[d].f[x], H[x]);
+        s += p;
+    }
+    return s;
+}
+
 struct Grid {
     int nx, ny, nz;
     // unknown index of (dir, node) or -1; node = i + (nx+1)*(j + (ny+1)*k)
@@ -239,6 +270,142 @@ double t_value(const ssv_params* p, int d, int i, int j, int k) {
     else f = (p->r_out - r) / (p->r_out - p->r_in);
     const double tz = p->ampere_turns / (p->zc1 - p->zc0);
     return h * tz * f;
+}
+
+// ---------------------------------------------------------------- meshes
+// A hexahedral edge-element mesh as the reference example's mesh files hold
+// it (examples/VoxelFEM/VoxelMesh.cpp:38-97): node coordinates, edges as
+// (start node, end node) and elements as 12 edge indices.  Each element must
+// be an axis-aligned box; its edges may come in any order and orientation.
+
+struct MeshElem {
+    int box;          // index into the distinct box sizes
+    int8_t loc[12];   // this generator's local edge index of the file's edge m
+    int8_t sgn[12];   // +1 if the edge runs along +axis, -1 otherwise
+};
+
+struct MeshPlan {
+    std::vector<int32_t> sub;          // edge -> unknown (or -1)
+    int32_t n = 0;
+    std::vector<int64_t> adj_off;      // edge -> (element, file-local index) list
+    std::vector<int32_t> adj_el;
+    std::vector<int8_t> adj_m;
+    std::vector<MeshElem> el;
+    std::vector<double> Hs;            // 3 per distinct box
+    std::vector<double> K, M, C;       // 144 per distinct box
+};
+
+// 0 on success, 3 for an element that is not an axis-aligned box.
+int plan_mesh(int32_t n_nodes, const double* xyz, int32_t n_edges, const int32_t* en, int32_t n_el,
+              const int32_t* ee, const uint8_t* zero, bool need_geometry, MeshPlan& P) {
+    // unknown numbering: non-eliminated edges in file order (VoxelMesh.cpp:101-122)
+    P.sub.assign(n_edges, -1);
+    P.n = 0;
+    for (int32_t e = 0; e < n_edges; ++e)
+        if (!zero || !zero[e]) P.sub[e] = P.n++;
+    P.adj_off.assign((size_t)n_edges + 1, 0);
+    for (int64_t q = 0; q < 12 * (int64_t)n_el; ++q) {
+        if (ee[q] < 0 || ee[q] >= n_edges) return 3;
+        ++P.adj_off[ee[q] + 1];
+    }
+    for (int32_t e = 0; e < n_edges; ++e) P.adj_off[e + 1] += P.adj_off[e];
+    P.adj_el.resize(P.adj_off[n_edges]);
+    P.adj_m.resize(P.adj_off[n_edges]);
+    {
+        std::vector<int64_t> pos(P.adj_off.begin(), P.adj_off.end() - 1);
+        for (int32_t c = 0; c < n_el; ++c)          // elements in file order
+            for (int m = 0; m < 12; ++m) {
+                const int64_t q = pos[ee[12 * (int64_t)c + m]]++;
+                P.adj_el[q] = c;
+                P.adj_m[q] = (int8_t)m;
+            }
+    }
+    if (!need_geometry) return 0;
+    P.el.resize(n_el);
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+        for (int32_t c = 0; c < n_el; ++c) {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int m = 0; m < 12; ++m) {
+                const int32_t e = ee[12 * (int64_t)c + m];
+                for (int end = 0; end < 2; ++end) {
+                    const int32_t nd = en[2 * (int64_t)e + end];
+                    if (nd < 0 || nd >= n_nodes) { bad = 1; continue; }
+                    for (int x = 0; x < 3; ++x) {
+                        lo[x] = std::min(lo[x], xyz[3 * (int64_t)nd + x]);
+                        hi[x] = std::max(hi[x], xyz[3 * (int64_t)nd + x]);
+                    }
+                }
+            }
+            if (bad) continue;
+            double H[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+            if (!(H[0] > 0 && H[1] > 0 && H[2] > 0)) { bad = 1; continue; }
+            int used = 0;
+            for (int m = 0; m < 12; ++m) {
+                const int32_t e = ee[12 * (int64_t)c + m];
+                const double* a = xyz + 3 * (int64_t)en[2 * (int64_t)e];
+                const double* b = xyz + 3 * (int64_t)en[2 * (int64_t)e + 1];
+                int d = 0;
+                for (int x = 1; x < 3; ++x)
+                    if (std::fabs(b[x] - a[x]) > std::fabs(b[d] - a[d])) d = x;
+                // an element edge spans its box along d and is a point across it
+                const double tol = 1e-9 * (H[0] + H[1] + H[2]);
+                bool ok = std::fabs(std::fabs(b[d] - a[d]) - H[d]) <= tol;
+                int off[2], k = 0;
+                for (int x = 0; x < 3; ++x) {
+                    if (x == d) continue;
+                    ok = ok && std::fabs(b[x] - a[x]) <= tol;
+                    off[k++] = a[x] > lo[x] + 0.5 * H[x] ? 1 : 0;
+                }
+                const int l = 4 * d + off[0] + 2 * off[1];
+                ok = ok && !(used & (1 << l));
+                used |= 1 << l;
+                if (!ok) { bad = 1; break; }
+                P.el[c].loc[m] = (int8_t)l;
+                P.el[c].sgn[m] = (int8_t)(b[d] > a[d] ? 1 : -1);
+            }
+            if (bad) continue;
+            P.el[c].box = -1;   // resolved below, serially, in element order
+        }
+    if (bad) return 3;
+    // distinct box sizes, in order of first appearance (deterministic)
+    for (int32_t c = 0; c < n_el; ++c) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int m = 0; m < 12; ++m) {
+            const int32_t e = ee[12 * (int64_t)c + m];
+            for (int end = 0; end < 2; ++end) {
+                const int32_t nd = en[2 * (int64_t)e + end];
+                for (int x = 0; x < 3; ++x) {
+                    lo[x] = std::min(lo[x], xyz[3 * (int64_t)nd + x]);
+                    hi[x] = std::max(hi[x], xyz[3 * (int64_t)nd + x]);
+                }
+            }
+        }
+        const double H[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        int id = -1;
+        for (size_t q = 0; q < P.Hs.size() / 3; ++q)
+            if (P.Hs[3 * q] == H[0] && P.Hs[3 * q + 1] == H[1] && P.Hs[3 * q + 2] == H[2]) { id = (int)q; break; }
+        if (id < 0) {
+            id = (int)(P.Hs.size() / 3);
+            P.Hs.insert(P.Hs.end(), H, H + 3);
+        }
+        P.el[c].box = id;
+    }
+    const size_t nb = P.Hs.size() / 3;
+    P.K.resize(144 * nb);
+    P.M.resize(144 * nb);
+    P.C.resize(144 * nb);
+    for (size_t q = 0; q < nb; ++q) {
+        VField nf[12], cf[12];
+        for (int l = 0; l < 12; ++l) { nf[l] = basis_box(l, &P.Hs[3 * q]); cf[l] = curl_box(l, &P.Hs[3 * q]); }
+        for (int a = 0; a < 12; ++a)
+            for (int b = 0; b < 12; ++b) {
+                P.K[144 * q + 12 * a + b] = inner_box(cf[a], cf[b], &P.Hs[3 * q]);
+                P.M[144 * q + 12 * a + b] = inner_box(nf[a], nf[b], &P.Hs[3 * q]);
+                P.C[144 * q + 12 * a + b] = inner_box(cf[a], nf[b], &P.Hs[3 * q]);
+            }
+    }
+    return 0;
 }
 
 }  // namespace
@@ -384,6 +551,122 @@ int ssv_build(const ssv_params* p, int complex_values, int32_t* row_ptr, int32_t
             double sre = 0.0, sim = 0.0;
             const int32_t col = ent[e].col;
             while (e < cnt && ent[e].col == col) { sre += ent[e].re; sim += ent[e].im; ++e; }
+            col_idx[pos] = col;
+            if (complex_values) { values[2 * pos] = sre; values[2 * pos + 1] = sim; }
+            else values[pos] = sre;
+            ++pos;
+        }
+        if (complex_values) { rhs[2 * r] = b_re; rhs[2 * r + 1] = 0.0; }
+        else rhs[r] = b_re;
+    }
+    return 0;
+}
+
+void ssv_box_element_matrices(double hx, double hy, double hz, double* K, double* M, double* C) {
+    const double H[3] = {hx, hy, hz};
+    VField n[12], cn[12];
+    for (int l = 0; l < 12; ++l) { n[l] = basis_box(l, H); cn[l] = curl_box(l, H); }
+    for (int a = 0; a < 12; ++a)
+        for (int b = 0; b < 12; ++b) {
+            if (K) K[12 * a + b] = inner_box(cn[a], cn[b], H);
+            if (M) M[12 * a + b] = inner_box(n[a], n[b], H);
+            if (C) C[12 * a + b] = inner_box(cn[a], n[b], H);
+        }
+}
+
+int ssv_mesh_count(int32_t n_edges, const int32_t* edge_nodes, int32_t n_elements, const int32_t* element_edges,
+                   const uint8_t* zero_edge, int32_t* n_unknowns, int64_t* nnz) {
+    if (n_edges <= 0 || n_elements < 0 || !edge_nodes || (n_elements && !element_edges)) return 1;
+    MeshPlan P;
+    const int rc = plan_mesh(0, nullptr, n_edges, edge_nodes, n_elements, element_edges, zero_edge, false, P);
+    if (rc) return rc;
+    int64_t total = 0;
+#pragma omp parallel for schedule(static) reduction(+ : total)
+    for (int32_t e = 0; e < n_edges; ++e) {
+        if (P.sub[e] < 0) continue;
+        std::vector<int32_t> cols;
+        for (int64_t q = P.adj_off[e]; q < P.adj_off[e + 1]; ++q)
+            for (int m = 0; m < 12; ++m) {
+                const int32_t u = P.sub[element_edges[12 * (int64_t)P.adj_el[q] + m]];
+                if (u >= 0) cols.push_back(u);
+            }
+        std::sort(cols.begin(), cols.end());
+        total += std::unique(cols.begin(), cols.end()) - cols.begin();
+    }
+    if (n_unknowns) *n_unknowns = P.n;
+    if (nnz) *nnz = total;
+    return 0;
+}
+
+int ssv_mesh_build(int32_t n_nodes, const double* xyz, int32_t n_edges, const int32_t* edge_nodes,
+                   int32_t n_elements, const int32_t* element_edges, const double* nu, const double* sigma,
+                   double omega, const double* t_edge, const uint8_t* zero_edge, int complex_values,
+                   int32_t* row_ptr, int32_t* col_idx, double* values, double* rhs) {
+    if (n_nodes <= 0 || !xyz || n_edges <= 0 || !edge_nodes || n_elements < 0 || (n_elements && !element_edges) ||
+        !nu || !row_ptr || !col_idx || !values || !rhs)
+        return 1;
+    MeshPlan P;
+    const int rc = plan_mesh(n_nodes, xyz, n_edges, edge_nodes, n_elements, element_edges, zero_edge, true, P);
+    if (rc) return rc;
+    std::vector<int32_t> row_edge(P.n);
+    for (int32_t e = 0; e < n_edges; ++e)
+        if (P.sub[e] >= 0) row_edge[P.sub[e]] = e;
+    struct Ent { int32_t col; int seq; double re, im; };
+    // pass 1: row lengths
+    std::vector<int32_t> len(P.n);
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < P.n; ++r) {
+        const int32_t e = row_edge[r];
+        std::vector<int32_t> cols;
+        for (int64_t q = P.adj_off[e]; q < P.adj_off[e + 1]; ++q)
+            for (int m = 0; m < 12; ++m) {
+                const int32_t u = P.sub[element_edges[12 * (int64_t)P.adj_el[q] + m]];
+                if (u >= 0) cols.push_back(u);
+            }
+        std::sort(cols.begin(), cols.end());
+        len[r] = (int32_t)(std::unique(cols.begin(), cols.end()) - cols.begin());
+    }
+    int64_t acc = 0;
+    row_ptr[0] = 0;
+    for (int32_t r = 0; r < P.n; ++r) {
+        acc += len[r];
+        if (acc > 2147483647LL) return 2;
+        row_ptr[r + 1] = (int32_t)acc;
+    }
+    // pass 2: values, contributions summed in element order then file-local order
+#pragma omp parallel for schedule(static)
+    for (int32_t r = 0; r < P.n; ++r) {
+        const int32_t e = row_edge[r];
+        std::vector<Ent> ent;
+        double b_re = 0.0;
+        for (int64_t q = P.adj_off[e]; q < P.adj_off[e + 1]; ++q) {
+            const int32_t c = P.adj_el[q];
+            const MeshElem& E = P.el[c];
+            const int mr = P.adj_m[q];
+            const int lr = E.loc[mr];
+            const double* K = &P.K[144 * E.box];
+            const double* Mm = &P.M[144 * E.box];
+            const double* Cm = &P.C[144 * E.box];
+            const double sg = sigma ? sigma[c] : 0.0;
+            for (int m = 0; m < 12; ++m) {
+                const int32_t em = element_edges[12 * (int64_t)c + m];
+                const int lm = E.loc[m];
+                const double s = (double)(E.sgn[mr] * E.sgn[m]);
+                if (t_edge && t_edge[em] != 0.0) b_re += s * Cm[12 * lm + lr] * t_edge[em];
+                const int32_t u = P.sub[em];
+                if (u < 0) continue;
+                ent.push_back(Ent{u, (int)ent.size(), s * nu[c] * K[12 * lr + lm],
+                                  complex_values ? s * omega * sg * Mm[12 * lr + lm] : 0.0});
+            }
+        }
+        std::sort(ent.begin(), ent.end(), [](const Ent& x, const Ent& y) {
+            return x.col != y.col ? x.col < y.col : x.seq < y.seq;
+        });
+        int64_t pos = row_ptr[r];
+        for (size_t k = 0; k < ent.size();) {
+            double sre = 0.0, sim = 0.0;
+            const int32_t col = ent[k].col;
+            while (k < ent.size() && ent[k].col == col) { sre += ent[k].re; sim += ent[k].im; ++k; }
             col_idx[pos] = col;
             if (complex_values) { values[2 * pos] = sre; values[2 * pos + 1] = sim; }
             else values[pos] = sre;

@@ -65,14 +65,20 @@ CASES = [("iccg", capi.ICCG, 1), ("mrtr_ic", capi.MRTR_IC, 0), ("mrtr_sgs", capi
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("sweep_mode", ["auto", "level"])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_ordered_solve_is_reference_algorithm_on_permuted_system(torch_cuda, ctx, case, cplx):
+def test_ordered_solve_is_reference_algorithm_on_permuted_system(torch_cuda, ctx, case, cplx, sweep_mode,
+                                                                 monkeypatch):
+    """sweep_mode "level" forces the one-launch-per-level sweeps (chosen by
+    itself for wide levels); both give the oracle's bits."""
     from oracle import oracle as O
+    if sweep_mode != "auto":
+        monkeypatch.setenv("SSGPU_SWEEP_MODE", sweep_mode)
     name, code, pt = case
     s = voxel.eddy_current(8, seed=5) if cplx else voxel.curlcurl(10, seed=5)
     bs = 8
-    print(f"abmc {name} cplx={cplx} N={s.n} bs={bs}", flush=True)
+    print(f"abmc {name} cplx={cplx} N={s.n} bs={bs} sweeps={sweep_mode}", flush=True)
     m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=bs)
     perm, nc = capi.abmc_order(s.row_ptr, s.col_idx, bs)
     assert m.ncolors == nc
@@ -138,3 +144,28 @@ def test_ordered_factored_block_and_update_values(torch_cuda, ctx):
     c2.close()
     m2.close()
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_level_sweeps_on_natural_order_bit_exact(torch_cuda, ctx, case, cplx, monkeypatch):
+    """The one-launch-per-level sweeps on the matrix's own order (many narrow
+    levels) return the oracle's bits too."""
+    from oracle import oracle as O
+    monkeypatch.setenv("SSGPU_SWEEP_MODE", "level")
+    name, code, pt = case
+    s = voxel.eddy_current(7, seed=6) if cplx else voxel.curlcurl(9, seed=6)
+    print(f"level sweeps {name} cplx={cplx} N={s.n}", flush=True)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    normB = O.norm(s.rhs)
+    conv, x, log, _ = m.solve(code, s.rhs, conv_cri=1e-8, max_ite=3000, normB=normB,
+                              opts=capi.default_options(is_save_log=1, accel_factor=1.2))
+    m.close()
+    solver = "iccg" if name == "iccg" else "mrtr"
+    with O.reduction_order(1):
+        c_r, x_r, log_r = O.solve(solver, s.row_ptr, s.col_idx, s.values, s.rhs, precondition_type=pt,
+                                  conv_cri=1e-8, max_ite=3000, normB=normB, accel_factor=1.2)
+    assert conv == c_r
+    assert np.array_equal(log, log_r)
+    assert np.array_equal(x, x_r)

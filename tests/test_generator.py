@@ -169,3 +169,26 @@ def test_coil_seed_changes_only_the_rhs():
     assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col_idx, b.col_idx)
     assert np.array_equal(a.values, b.values)
     assert np.linalg.norm(a.rhs - b.rhs) > 0.1 * np.linalg.norm(a.rhs)
+
+
+@pytest.mark.parametrize("hx,hy,hz", [(1.0, 2.0, 0.5), (0.001, 0.01, 0.001), (0.3, 0.3, 0.07)])
+def test_box_element_matrix_matches_reference_example(hx, hy, hz):
+    """Anisotropic boxes: the example's closed form takes three half lengths
+    (VoxelMatrix.cpp:133-136); the graded mesh of the example has such boxes."""
+    K, M, C = voxel.box_element_matrices(hx, hy, hz)
+    v = [1.0, 0, 0, 0, 1.0, 0, 0, 0, 1.0]
+    for k in range(12):
+        for m in range(k, 12):
+            ref = _voxelfem_entry(k, m, v, hx / 2.0, hy / 2.0, hz / 2.0)
+            mine = K[_their_to_mine(k), _their_to_mine(m)]
+            assert abs(mine - ref) <= 1e-12 * max(1.0, abs(ref)), (k, m, mine, ref)
+    # the RHS coupling int N_k . curl N_m = +-1/8 does not depend on the box (VoxelMatrix.cpp:53-108)
+    _, _, C1 = voxel.element_matrices(1.0)
+    np.testing.assert_allclose(C, C1, rtol=0, atol=1e-15)
+    # mass matrix: int N_k . N_m of two parallel edges = (area / edge length) x (1/3 or 1/6 per transverse axis)
+    assert abs(M[0, 0] - (hy * hz / hx) / 9.0) <= 1e-15 * M[0, 0]
+    assert abs(M[0, 3] - (hy * hz / hx) / 36.0) <= 1e-15 * M[0, 0]
+    # equal sizes reduce to the cube
+    Kc, Mc, Cc = voxel.element_matrices(hx)
+    Kb, Mb, Cb = voxel.box_element_matrices(hx, hx, hx)
+    assert np.array_equal(Kc, Kb) and np.array_equal(Mc, Mb) and np.array_equal(Cc, Cb)

@@ -180,3 +180,31 @@ def test_pybind_solve_matches_oracle(cplx, kind, torch_cuda):
     tol = _sensitive_tol(cplx, kind)
     assert np.linalg.norm(x - xr) <= tol * max(np.linalg.norm(xr), 1e-300)
     assert normB > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cplx", [False, True])
+def test_pybind_abmc_ordering(cplx, torch_cuda):
+    """setABMC(b): the same system solved on its ABMC reordering converges,
+    in the caller's order, to a solution of the same system.  The curl-curl
+    systems are singular (no gauge), so solutions may differ by a gradient:
+    x is compared through A x."""
+    m = _module()
+    s = voxel.eddy_current(6, seed=3) if cplx else voxel.curlcurl(8, seed=3)
+    A = _sparse_mat(m, s)
+    fn = "solve_complex" if cplx else "solve_real"
+    x0 = np.zeros(s.n, dtype=s.rhs.dtype)
+    nat = m.MatSolverICCG()
+    nat.setAccelFactor(1.1)
+    ok_n, x_n = getattr(nat, fn)(1e-10, 3000, A, s.rhs, x0)
+    abmc = m.MatSolverICCG()
+    abmc.setAccelFactor(1.1)
+    abmc.setABMC(4)
+    abmc.setSaveLog(True)
+    ok_a, x_a = getattr(abmc, fn)(1e-10, 3000, A, s.rhs, x0)
+    assert ok_n and ok_a
+    A_sp = s.scipy()
+    nb = np.linalg.norm(s.rhs)
+    assert np.linalg.norm(s.rhs - A_sp @ x_a) <= 1e-8 * nb
+    assert np.linalg.norm(A_sp @ (x_a - x_n)) <= 2e-8 * nb
+    assert len(abmc.getResidualLog()) - 1 == abmc.getIterations()
