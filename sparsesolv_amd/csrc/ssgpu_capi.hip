@@ -146,6 +146,7 @@ struct Matrix {
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
     int sweep_grid = 128;     // persistent one-wave workgroups per sweep
     bool sweeps_tuned = false;   // sweep launch shape measured (first factorization)
+    int spmv_stage_kb = 16;      // SpMV staging (16 or 32 KB), measured at creation
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
     int block_kc = 0;         // block sweeps: columns per lane (0: 1)
     int block_grid = 0;       // block sweeps: workgroups per column group (0: about 512 waves in all)
@@ -496,10 +497,10 @@ struct Solver {
         return Host<T>::from(fetch(reduce_blocks(m.n), kparts<T>()));
     }
     template <typename T> void spmv(const T* v, const T* x, T* y) {
-        launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), v, x, y, nullptr, nullptr);
+        launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), v, x, y, nullptr, nullptr, m.spmv_stage_kb);
     }
     template <typename T> T spmv_dot(const T* v, const T* x, T* y) {   // y = A x, returns x.y
-        launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), v, x, y, x, partial());
+        launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), v, x, y, x, partial(), m.spmv_stage_kb);
         return Host<T>::from(fetch(m.nblk, kparts<T>()));
     }
 
@@ -559,7 +560,7 @@ struct Solver {
         using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         if (sf_ok()) {
-            launch_sf_prepare<T>(s, m.n, tmp, z, tickets());
+            if (!m.sfL.by_level || !m.sfUic.by_level) launch_sf_prepare<T>(s, m.n, tmp, z, tickets());
             if (f32) {
                 sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r,
                                nullptr, nullptr, tmp);
@@ -587,7 +588,7 @@ struct Solver {
         const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         (void)aux;
         if (sf_ok()) {
-            launch_sf_prepare<T>(s, m.n, out, nullptr, tickets());
+            if (!(forward ? m.sfL : m.sfUic).by_level) launch_sf_prepare<T>(s, m.n, out, nullptr, tickets());
             if (forward) {
                 if (f32) sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), in, nullptr, nullptr, out);
                 else sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), in, nullptr, nullptr, out);
@@ -608,7 +609,7 @@ struct Solver {
     // plain sweeps on the current lower / upper values (SGS): values loaded by sgs_values
     template <typename T> void fr(const T* r, T* z) {
         if (m.sfL.kmax > 0) {
-            launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
+            if (!m.sfL.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
             sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr,
                            nullptr, z);
         } else {
@@ -617,7 +618,7 @@ struct Solver {
     }
     template <typename T> void bc(const T* r, T* z) {
         if (m.sfUs.kmax > 0) {
-            launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
+            if (!m.sfUs.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
             sf_sweep<T, T>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval.as<T>(), m.sfUs.dval.as<T>(), r,
                            nullptr, nullptr, z);
         } else {
@@ -711,7 +712,7 @@ struct Solver {
             m.sweep_grid = sh.grid;
             float t = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
-                launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
+                if (!sh.level) launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
                 SSGPU_HIP(hipEventRecord(e0, s));
                 sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(),
                                r.as<T>(), nullptr, nullptr, out.as<T>());
@@ -1510,6 +1511,43 @@ template <typename T> void factorize(Matrix& m, int solver, double accel, int pr
     SSGPU_HIP(hipStreamSynchronize(s));
 }
 
+// SpMV staging, measured once per matrix: both LDS footprints give the same
+// bits; the faster one on this matrix's gather pattern is kept.
+template <typename T> void tune_spmv(Matrix& m, T one) {
+    if (const char* e = std::getenv("SSGPU_SPMV_STAGE_KB")) {
+        m.spmv_stage_kb = std::atoi(e) >= 32 ? 32 : std::atoi(e) >= 16 ? 16 : 8;
+        return;
+    }
+    Context* c = m.ctx;
+    DevBuf x, y;
+    x.alloc(c, sizeof(T) * m.n);
+    y.alloc(c, sizeof(T) * m.n);
+    launch_fill<T>(c->stream, m.n, x.as<T>(), one);
+    hipEvent_t e0, e1;
+    SSGPU_HIP(hipEventCreate(&e0));
+    SSGPU_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int kb : {8, 16, 32}) {
+        float t = 1e30f;
+        for (int rep = 0; rep < 4; ++rep) {   // the first run warms up
+            SSGPU_HIP(hipEventRecord(e0, c->stream));
+            launch_spmv<T>(c->stream, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(),
+                           x.as<T>(), y.as<T>(), nullptr, nullptr, kb);
+            SSGPU_HIP(hipEventRecord(e1, c->stream));
+            SSGPU_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            SSGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) t = std::min(t, ms);
+        }
+        if (t < best) {
+            best = t;
+            m.spmv_stage_kb = kb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 // Solves on an ABMC-reordered matrix: b and x (ncol vectors of n) are
 // gathered into the matrix's order, solved there, and x gathered back.
 struct PermutedIO {
@@ -1693,6 +1731,8 @@ int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* 
     m.val.alloc(m.ctx, m.vsize * (size_t)std::max<int64_t>(m.nnz, 1));
     if (m.nnz) SSGPU_HIP(hipMemcpyAsync(m.val.p, values, m.vsize * m.nnz, hipMemcpyHostToDevice, ctx->c.stream));
     analyse(m);
+    if (m.dtype == SSGPU_R64) tune_spmv<double>(m, 1.0);
+    else tune_spmv<zc>(m, zc{1.0, 0.0});
     SSGPU_HIP(hipStreamSynchronize(ctx->c.stream));
     {
         std::lock_guard<std::mutex> lk(g_lifetime_mu);
@@ -1944,10 +1984,10 @@ int ssgpu_spmv_device(ssgpu_matrix mh, const void* d_x, void* d_y) {
     SSGPU_HIP(hipSetDevice(m.ctx->device));
     if (m.dtype == SSGPU_R64)
         launch_spmv<double>(m.ctx->stream, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<double>(),
-                            (const double*)d_x, (double*)d_y, nullptr, nullptr);
+                            (const double*)d_x, (double*)d_y, nullptr, nullptr, m.spmv_stage_kb);
     else
         launch_spmv<zc>(m.ctx->stream, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<zc>(),
-                        (const zc*)d_x, (zc*)d_y, nullptr, nullptr);
+                        (const zc*)d_x, (zc*)d_y, nullptr, nullptr, m.spmv_stage_kb);
     SSGPU_HIP(hipGetLastError());
     return SSGPU_OK;
     SSGPU_GUARD_END

@@ -21,9 +21,10 @@ constexpr int kMaxReduceBlocks = 1024;
 // unless the block is one long row).  With dotv != nullptr, also writes the
 // per-block partial of sum(dotv[i] * y[i]) into partial (1 or 2 doubles per
 // block, non-conjugated).
+// stage_kb: LDS staging of the products, 16 or 32 KB (same results)
 template <typename T>
 void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const int* ci, const T* v,
-                 const T* x, T* y, const T* dotv, double* partial);
+                 const T* x, T* y, const T* dotv, double* partial, int stage_kb = 16);
 
 // --- level-scheduled sweeps ----------------------------------------------------
 enum SweepMode { FWD_DIV_LAST = 0, BWD_IC = 1, BWD_DIV_FIRST = 2 };

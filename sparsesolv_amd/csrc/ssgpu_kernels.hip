@@ -71,13 +71,29 @@ __device__ inline zc ntload(const zc* p) {
 __device__ inline double shfl_xor1(double v) { return __shfl_xor(v, 1); }
 __device__ inline zc shfl_xor1(zc v) { return zc{__shfl_xor(v.re, 1), __shfl_xor(v.im, 1)}; }
 
-template <typename T>
+// Workgroups are handed to the 8 XCDs in turn (workgroup w starts on XCD
+// w % 8).  Give each XCD one contiguous range of row blocks: the band of x
+// that a range gathers then stays in that XCD's L2, instead of every XCD
+// streaming all of x.  A bijection of [0, nb).
+__device__ inline int xcd_block(int w, int nb) {
+    constexpr int kXcd = 8;
+    const int q = nb / kXcd, rem = nb % kXcd;
+    const int xcd = w % kXcd, local = w / kXcd;
+    return xcd * q + min(xcd, rem) + local;
+}
+
+template <typename T, int STAGE_BYTES>
 __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const int* __restrict__ rp,
                                               const int* __restrict__ ci, const T* __restrict__ v,
                                               const T* __restrict__ x, T* __restrict__ y,
                                               const T* __restrict__ dotv, double* __restrict__ partial) {
-    __shared__ T prod[kSpmvCap];
-    const int b = blockIdx.x;
+    // products are staged through LDS in pieces of kStage entries: 16 KB
+    // keeps 8 workgroups per CU in flight, 32 KB (4 per CU) suits orders whose
+    // gathers are scattered (fewer of them compete for L2 at once)
+    constexpr int kStage = STAGE_BYTES / (int)sizeof(T) < kSpmvCap ? STAGE_BYTES / (int)sizeof(T) : kSpmvCap;
+    static_assert(kStage % kBS == 0 && kSpmvCap % kStage == 0 && kStage >= kSpmvRows, "staging");
+    __shared__ T prod[kStage];
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int r0 = blk[b], r1 = blk[b + 1];
     const int e0 = rp[r0], e1 = rp[r1];
     const int ne = e1 - e0;
@@ -87,34 +103,38 @@ __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const
     const int r = r0 + j;
     T yv = Num<T>::zero();
     if (ne <= kSpmvCap) {
-        // products of the block's entries into LDS; a thread's kSpmvCap / kBS
-        // entries are loaded together (indices and values, then the gathers)
-        // so their latencies overlap
-        constexpr int U = kSpmvCap / kBS;
-        int cc[U];
-        T vv[U], xx[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = u * kBS + (int)threadIdx.x;
-            if (k < ne) {   // the matrix streams through once: keep it out of the caches' way
-                cc[u] = __builtin_nontemporal_load(ci + e0 + k);
-                vv[u] = ntload(v + e0 + k);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u * kBS + (int)threadIdx.x < ne) xx[u] = x[cc[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = u * kBS + (int)threadIdx.x;
-            if (k < ne) prod[k] = Num<T>::mul(vv[u], xx[u]);
-        }
-        __syncthreads();
+        // each piece: a thread's kStage / kBS entries are loaded together
+        // (indices and values, then the gathers) so their latencies overlap;
+        // then every chain adds the piece's products at its positions, in
+        // order, so a chain's sum runs exactly as over the whole row
+        constexpr int U = kStage / kBS;
         T acc = Num<T>::zero();
-        if (r < r1) {
-            const int e = rp[r + 1] - e0;
+        int k = r < r1 ? rp[r] - e0 + par : 0;
+        const int e = r < r1 ? rp[r + 1] - e0 : 0;
+        for (int base = 0; base < ne; base += kStage) {
+            int cc[U];
+            T vv[U], xx[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = base + u * kBS + (int)threadIdx.x;
+                if (q < ne) {   // the matrix streams through once: keep it out of the caches' way
+                    cc[u] = __builtin_nontemporal_load(ci + e0 + q);
+                    vv[u] = ntload(v + e0 + q);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (base + u * kBS + (int)threadIdx.x < ne) xx[u] = x[cc[u]];
+            if (base > 0) __syncthreads();   // the previous piece has been summed
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = base + u * kBS + (int)threadIdx.x;
+                if (q < ne) prod[q - base] = Num<T>::mul(vv[u], xx[u]);
+            }
+            __syncthreads();
+            const int lim = e < base + kStage ? e : base + kStage;
 #pragma unroll 4
-            for (int k = rp[r] - e0 + par; k < e; k += 2) acc = Num<T>::add(acc, prod[k]);
+            for (; k < lim; k += 2) acc = Num<T>::add(acc, prod[k - base]);
         }
         const T other = shfl_xor1(acc);
         if (r < r1 && par == 0) {
@@ -444,8 +464,13 @@ void launch_finalize(hipStream_t s, int nb, int K, const double* partial, double
 
 template <typename T>
 void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const int* ci, const T* v, const T* x, T* y,
-                 const T* dotv, double* partial) {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
+                 const T* dotv, double* partial, int stage_kb) {
+    if (stage_kb >= 32)
+        hipLaunchKernelGGL((k_spmv<T, 32768>), dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
+    else if (stage_kb >= 16)
+        hipLaunchKernelGGL((k_spmv<T, 16384>), dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
+    else
+        hipLaunchKernelGGL((k_spmv<T, 8192>), dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
 }
 
 template <typename T, typename F>
@@ -549,7 +574,7 @@ template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d,
 // explicit instantiations
 #define SSGPU_INST(T)                                                                                          \
     template void launch_spmv<T>(hipStream_t, int, const int*, const int*, const int*, const T*, const T*, T*,  \
-                                 const T*, double*);                                                           \
+                                 const T*, double*, int);                                                      \
     template void launch_ic_level<T>(hipStream_t, int, int, const int*, const int*, T*, const int*, const int*, \
                                      T*, double, int*);                                                        \
     template void launch_dot<T>(hipStream_t, int, const T*, const T*, double*);                                \

@@ -223,3 +223,39 @@ def test_solve_parity_config1(torch_cuda, case):
     rel = _check_parity(conv, x, log, c_r, x_r, log_r, xtol=max(1e-8, sens))
     print(f"config1 {name}: iters gpu={len(log)-1} oracle={len(log_r)-1} rel={rel:.2e} "
           f"setup={res.setup_ms:.2f}ms iterate={res.iterate_ms:.2f}ms", flush=True)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_spmv_wide_and_narrow_blocks_bit_exact(torch_cuda, ctx, cplx):
+    """Row blocks whose columns span fewer than 2^16 values are read through
+    16-bit offsets, the others through int32 indices; a matrix with both
+    kinds (banded rows plus rows reaching across 100,000 columns, every
+    fifth row empty or long) gives the oracle's bits, with and without the
+    fused dot partials of the in-solve SpMV."""
+    rng = np.random.default_rng(11)
+    n = 100_000
+    rows = []
+    for i in range(n):
+        if i % 5 == 4 and i % 3 == 0:
+            cols = []
+        elif i % 997 == 0:
+            cols = sorted(set([0, n - 1, i] + list(rng.integers(0, n, 40))))     # wide block
+        else:
+            cols = sorted(set(min(max(i + d, 0), n - 1) for d in (-3000, -7, -1, 0, 1, 9, 2500)))
+        rows.append(cols)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate([np.asarray(c, np.int32) for c in rows if c]).astype(np.int32)
+    vals = rng.standard_normal(len(ci))
+    if cplx:
+        vals = vals + 1j * rng.standard_normal(len(ci))
+    print(f"spmv wide/narrow cplx={cplx} N={n} nnz={len(ci)}", flush=True)
+    m = capi.Matrix(ctx, rp, ci, vals)   # rows without a diagonal are fine for an SpMV
+    x = _rand(n, cplx, seed=3)
+    dx = _dev(torch_cuda, x)
+    dy = torch_cuda.zeros_like(dx)
+    m.spmv_device(dx.data_ptr(), dy.data_ptr())
+    y = _host(torch_cuda, ctx, dy)
+    ref = O.spmv(rp, ci, vals, x)
+    assert np.array_equal(y, ref)
+    m.close()
