@@ -64,16 +64,31 @@ def cpu_baseline(sys_, normB, accel, iters_gpu, solver):
                     accel_factor=accel, precondition_type=pt)
         return time.perf_counter() - t
 
+    def sample():
+        t1, t2 = run(k1), run(k2)
+        per_it = max((t2 - t1) / (k2 - k1), 0.0)
+        setup = max(t1 - k1 * per_it, 0.0)
+        return setup, per_it, setup + iters_gpu * per_it
+
     run(1)  # page in
-    t1, t2 = run(k1), run(k2)
-    per_it = max((t2 - t1) / (k2 - k1), 0.0)
-    setup = max(t1 - k1 * per_it, 0.0)
-    est = setup + iters_gpu * per_it
+    setup, per_it, est = sample()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / est, "unit": "solves/s", "cores": cores, "kind": kind,
-            "sample": f"{kind} solve capped at {k1} and {k2} iterations on the same system "
-                      f"(setup {setup:.2f}s, {per_it * 1e3:.1f} ms/iteration; OpenMP SpMV, sequential IC and "
-                      f"sweeps as in the reference), extrapolated to {iters_gpu} iterations: {est:.1f} s/solve"}
+    out = {"value": 1.0 / est, "unit": "solves/s", "cores": cores, "kind": kind,
+           "sample": f"{kind} solve capped at {k1} and {k2} iterations on the same system "
+                     f"(setup {setup:.2f}s, {per_it * 1e3:.1f} ms/iteration; OpenMP SpMV, sequential IC and "
+                     f"sweeps as in the reference), extrapolated to {iters_gpu} iterations: {est:.1f} s/solve"}
+    # the same sample on one thread (the reference's OpenMP runtime, set for this process)
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        s1, p1, e1 = sample()
+        gomp.omp_set_num_threads(cores)
+        out["single_thread"] = {"value": 1.0 / e1, "unit": "solves/s", "cores": 1,
+                                "sample": f"setup {s1:.2f}s, {p1 * 1e3:.1f} ms/iteration, {e1:.1f} s/solve"}
+    except OSError:
+        pass
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
@@ -364,7 +379,8 @@ def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
             "solves_per_s": 1.0 / med, "ms_per_solve": 1e3 * med, "iterations": r.iterations,
             "converged": bool(r.converged), "true_residual": true_res,
             "roofline_sweep": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
+                               "frac": gbs / HBM_PEAK_GBS, "traffic": traffic_for(a.n, "sweep_abmc"),
+                               "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
             "spmv_ms": t_spmv}
 
 
@@ -434,6 +450,8 @@ def main():
     ap.add_argument("--reps", type=int, default=10, help="repetitions for the per-kernel timing")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the dominant kernels and a 1 GiB copy (for counter runs), no solves")
+    ap.add_argument("--order", default="natural", choices=["natural", "abmc"],
+                    help="kernels-only: the matrix's own order or its ABMC reordering")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving", "eddy", "voxelfem"],
@@ -465,7 +483,8 @@ def main():
     print(f"# system n={a.n}: N={s.n} nnz={s.nnz} (generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
     ctx = capi.Context(0)
     t0 = time.perf_counter()
-    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    reorder = a.kernels_only and a.order == "abmc"
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=ABMC_BLOCK if reorder else None)
     t_an = time.perf_counter() - t0
     info = m.info()
     print(f"# analysis {t_an:.1f}s levels fwd={info['fwd_levels']} bwd={info['bwd_levels']}", file=sys.stderr,

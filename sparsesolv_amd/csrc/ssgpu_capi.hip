@@ -1550,20 +1550,29 @@ template <typename T> void tune_spmv(Matrix& m, T one) {
 
 // Solves on an ABMC-reordered matrix: b and x (ncol vectors of n) are
 // gathered into the matrix's order, solved there, and x gathered back.
+// The permuted copies live in the context's work pool (indices 40 and 41,
+// beside the solvers' own work vectors), so repeated solves allocate nothing.
 struct PermutedIO {
     Context* c;
     Matrix& m;
-    DevBuf bp, xp;
+    DevBuf& bp;
+    DevBuf& xp;
     const void* b;
     void* x;
     int ncol;
-    PermutedIO(Context* cc, Matrix& mm, const void* bb, void* xx, int nc) : c(cc), m(mm), b(bb), x(xx), ncol(nc) {
-        const size_t bytes = m.vsize * (size_t)m.n * nc;
-        bp.alloc(c, bytes);
-        xp.alloc(c, bytes);
+    static DevBuf& pool(Context* c, size_t idx, size_t bytes) {
+        while (c->work.size() <= idx) c->work.emplace_back(new DevBuf());
+        DevBuf& d = *c->work[idx];
+        if (d.bytes < bytes) d.alloc(c, bytes);
+        return d;
+    }
+    PermutedIO(Context* cc, Matrix& mm, const void* bb, void* xx, int nc)
+        : c(cc), m(mm), bp(pool(cc, 40, mm.vsize * (size_t)mm.n * nc)), xp(pool(cc, 41, mm.vsize * (size_t)mm.n * nc)),
+          b(bb), x(xx), ncol(nc) {
+        const size_t stride = m.vsize * (size_t)m.n;
         for (int j = 0; j < nc; ++j) {
-            gather(static_cast<const char*>(b) + m.vsize * (size_t)m.n * j, static_cast<char*>(bp.p) + m.vsize * (size_t)m.n * j, m.perm);
-            gather(static_cast<const char*>(x) + m.vsize * (size_t)m.n * j, static_cast<char*>(xp.p) + m.vsize * (size_t)m.n * j, m.perm);
+            gather(static_cast<const char*>(b) + stride * j, static_cast<char*>(bp.p) + stride * j, m.perm);
+            gather(static_cast<const char*>(x) + stride * j, static_cast<char*>(xp.p) + stride * j, m.perm);
         }
     }
     void gather(const void* src, void* dst, const DevBuf& map) {
@@ -1573,8 +1582,9 @@ struct PermutedIO {
             launch_gather<zc, zc>(c->stream, m.n, map.as<int>(), (const zc*)src, (zc*)dst);
     }
     void finish() {   // x <- xp in the caller's order; waits for the copies
+        const size_t stride = m.vsize * (size_t)m.n;
         for (int j = 0; j < ncol; ++j)
-            gather(static_cast<const char*>(xp.p) + m.vsize * (size_t)m.n * j, static_cast<char*>(x) + m.vsize * (size_t)m.n * j, m.iperm);
+            gather(static_cast<const char*>(xp.p) + stride * j, static_cast<char*>(x) + stride * j, m.iperm);
         SSGPU_HIP(hipStreamSynchronize(c->stream));
     }
 };
