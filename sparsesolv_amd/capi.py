@@ -1,7 +1,7 @@
 """ctypes binding of the C-ABI in include/sparsesolv_amd.h.
 
-This is the same boundary a reference-side maintainer would bind (see
-INTEGRATION.md); the parity tests call the GPU path through it.  The library
+This is the same boundary a reference-side maintainer would bind from a
+ctypes / cffi stub; the parity tests call the GPU path through it.  The library
 must be the in-tree HIP build: if it is missing the import fails loudly, there
 is no CPU fallback.
 """
