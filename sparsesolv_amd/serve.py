@@ -12,8 +12,9 @@ sweeps carry all the right-hand sides through the dependency chain at once;
 requests that arrive while it runs wait for the next free slot.  Matrices
 stay resident with their preconditioner (the shifted IC(0) factor, or S A S
 for SGS-MRTR) in a cache keyed by (matrix key, solver family, shift, factor
-precision); a request for a cached system skips the upload, the analysis and
-the factorization.  The
+precision, ordering); a request for a cached system skips the upload, the
+analysis and the factorization.  A request may ask for the ABMC reordering
+of its system (abmc_block): faster sweeps, a different preconditioner.  The
 cache is bounded by an HBM budget: least recently used systems nobody is
 solving on are evicted to admit a new one, and a request waits while the
 budget is taken by systems in use.
@@ -51,6 +52,7 @@ class Request:
     conv_cri: float = 1e-8
     max_ite: int = 20000
     opts: dict = field(default_factory=dict)
+    abmc_block: Optional[int] = None  # solve on the ABMC reordering (not a parity mode; see capi.Matrix)
     arrival: float = 0.0             # perf_counter() at submission (set by submit)
 
 
@@ -145,7 +147,7 @@ class Engine:
     # ------------------------------------------------------------ internals
     def _cache_key(self, r: Request):
         family = "sgs" if r.solver == capi.MRTR_SGS else "ic"
-        return (r.key, family, None if family == "sgs" else float(r.accel), int(r.precision))
+        return (r.key, family, None if family == "sgs" else float(r.accel), int(r.precision), r.abmc_block)
 
     def _batch_key(self, r: Request):
         """Requests with equal keys can share one block solve."""
@@ -210,7 +212,7 @@ class Engine:
                 self.lock.wait()   # budget held by systems in use
         if not hit:
             try:
-                m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values)
+                m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values, abmc_block=r.abmc_block)
                 m.factorize(r.solver, r.accel, r.precision)
                 e.matrix = m
             except Exception as ex:   # noqa: BLE001 - reported to every waiter

@@ -217,3 +217,30 @@ def test_serving_batches_same_system_requests(torch_cuda, ctx):
         x_solo, r_solo = _solo(torch_cuda, ctx, s, b, capi.ICCG, 1.1)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo)
+
+
+def test_serving_abmc_requests(torch_cuda, ctx):
+    """A request that asks for the ABMC reordering is solved on it: its
+    response equals a solo solve on an ABMC-ordered matrix, and it does not
+    share a cache entry with the natural-order request for the same key."""
+    s = voxel.curlcurl(9, seed=9)
+    b = _rhs(s, 300)
+    eng = serve.Engine(slots=2)
+    futs = [eng.submit(serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                     accel=1.1, abmc_block=blk)) for blk in (None, 4)]
+    resp = [f.result(timeout=120) for f in futs]
+    eng.close()
+    assert all(r.error is None for r in resp)
+    assert not resp[1].cache_hit
+    x_nat, r_nat = _solo(torch_cuda, ctx, s, b, capi.ICCG, 1.1)
+    assert np.array_equal(resp[0].x, x_nat)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=4)
+    d_b = torch_cuda.from_numpy(b).cuda()
+    d_x = torch_cuda.zeros_like(d_b)
+    torch_cuda.cuda.synchronize()
+    r = m.solve_device(capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=5000,
+                       normB=float(np.linalg.norm(b)), opts=capi.default_options(accel_factor=1.1))
+    x_abmc = d_x.cpu().numpy()
+    m.close()
+    assert resp[1].result.iterations == r.iterations
+    assert np.array_equal(resp[1].x, x_abmc)
