@@ -19,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sparsesolv_amd.h"
@@ -95,6 +96,26 @@ template <typename T> void upload(Context* c, DevBuf& d, const std::vector<T>& h
     d.alloc(c, sizeof(T) * std::max<size_t>(h.size(), 1));
     // synchronous: h may be a temporary of the analysis
     if (!h.empty()) SSGPU_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+// Host loops of the analysis whose iterations write disjoint outputs: split
+// [0, n) into contiguous ranges over a few threads (results do not depend on
+// the split).
+template <typename F> void parallel_for(int n, F&& f) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({hw > 0 ? hw : 1, 16, n / 65536}));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+            const int a = (int)((int64_t)n * t / nt), b = (int)((int64_t)n * (t + 1) / nt);
+            for (int i = a; i < b; ++i) f(i);
+        });
+    for (auto& x : th) x.join();
 }
 
 // ------------------------------------------------------------------ structure
@@ -208,18 +229,18 @@ static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const
     }
     std::vector<int> prp(n + 1, 0), pci(rp[n]), prow(n), map(rp[n]);
     if (pos_out) pos_out->assign(n, 0);
-    int64_t e = 0;
-    for (int p = 0; p < n; ++p) {
+    for (int p = 0; p < n; ++p) prp[p + 1] = prp[p] + (rp[order[p] + 1] - rp[order[p]]);
+    parallel_for(n, [&](int p) {
         const int i = order[p];
         prow[p] = i;
         if (pos_out) (*pos_out)[i] = p;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        int e = prp[p];
+        for (int k = rp[i]; k < rp[i + 1]; ++k, ++e) {
             pci[e] = ci[k];
             map[e] = src[k];
-            ++e;
         }
-        prp[p + 1] = (int)e;
-    }
+    });
+    const int64_t e = prp[n];
     out.nlev = nlev;
     out.nnz = e;
     if (order_out) *order_out = order;
@@ -263,7 +284,7 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
     const int64_t nell = soff[nsl];
     std::vector<int> ecol(std::max<int64_t>(nell, 1), 0), ef(std::max<int64_t>(nell, 1), -1),
         ea(std::max<int64_t>(nell, 1), -1), df(n), da(n);
-    for (int p = 0; p < n; ++p) {
+    parallel_for(n, [&](int p) {
         const int i = order[p];
         const int a = rows.off[i], len = rows.len(i);
         rlen[p] = len;
@@ -275,7 +296,7 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
         }
         df[p] = rows.df[i];
         da[p] = rows.da[i];
-    }
+    });
     out.kmax = kmax <= 64 ? std::max(kmax, 1) : 0;
     out.nell = nell;
     out.srow = lv.prow.as<int>();
@@ -355,21 +376,28 @@ static void analyse(Matrix& m) {
     if (const char* g = std::getenv("SSGPU_BLOCK_KC")) m.block_kc = std::atoi(g);
     if (const char* g = std::getenv("SSGPU_BLOCK_GRID")) m.block_grid = std::max(1, std::atoi(g));
     std::vector<int> diag(n, -1);
-    for (int i = 0; i < n; ++i)
+    parallel_for(n, [&](int i) {
         for (int k = rp[i]; k < rp[i + 1]; ++k)
             if (ci[k] == i) { diag[i] = k; break; }
+    });
     upload(c, m.diag, diag);
     // lower(A): entries col <= row, order kept (SprsTrans.cpp:372-379)
-    std::vector<int> lrp(n + 1, 0), lci, lsrc;
-    lci.reserve((m.nnz + n) / 2);
-    lsrc.reserve((m.nnz + n) / 2);
-    m.lower_diag_ok = true;
-    for (int i = 0; i < n; ++i) {
+    std::vector<int> lrp(n + 1, 0);
+    parallel_for(n, [&](int i) {
+        int cnt = 0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) cnt += ci[k] <= i;
+        lrp[i + 1] = cnt;
+    });
+    for (int i = 0; i < n; ++i) lrp[i + 1] += lrp[i];
+    std::vector<int> lci(lrp[n]), lsrc(lrp[n]);
+    parallel_for(n, [&](int i) {
+        int q = lrp[i];
         for (int k = rp[i]; k < rp[i + 1]; ++k)
-            if (ci[k] <= i) { lci.push_back(ci[k]); lsrc.push_back(k); }
-        lrp[i + 1] = (int)lci.size();
+            if (ci[k] <= i) { lci[q] = ci[k]; lsrc[q] = k; ++q; }
+    });
+    m.lower_diag_ok = true;
+    for (int i = 0; i < n; ++i)
         if (lrp[i + 1] == lrp[i] || lci[lrp[i + 1] - 1] != i) m.lower_diag_ok = false;
-    }
     m.lmax_row = 0;
     for (int i = 0; i < n; ++i) m.lmax_row = std::max(m.lmax_row, lrp[i + 1] - lrp[i]);
     std::vector<int> pos, order_l, order_u;
@@ -412,19 +440,29 @@ static void analyse(Matrix& m) {
                 }
         rl.alloc();
         ru.alloc();
-        std::vector<int> fl(rl.off.begin(), rl.off.end() - 1), fu(ru.off.begin(), ru.off.end() - 1);
-        for (int i = 0; i < n; ++i) {   // ascending i: each transposed row fills in ascending order
+        parallel_for(n, [&](int i) {   // rows of lower(A): each row's own range
             const int a = lrp[i], e = lrp[i + 1];
+            int ql = rl.off[i];
             for (int k = a; k < e; ++k) {
                 const int j = lci[k];
                 const int f = lstart[i] + (k - a);
                 if (j < i) {
-                    const int ql = fl[i]++, qu = fu[j]++;
                     rl.col[ql] = j; rl.sf[ql] = f; rl.sa[ql] = lsrc[k];
-                    ru.col[qu] = i; ru.sf[qu] = f; ru.sa[qu] = -1;
+                    ++ql;
                 } else if (j == i && k == e - 1) {
                     rl.df[i] = f; rl.da[i] = lsrc[k];
                     ru.df[i] = f;
+                }
+            }
+        });
+        std::vector<int> fu(ru.off.begin(), ru.off.end() - 1);
+        for (int i = 0; i < n; ++i) {   // ascending i: each transposed row fills in ascending order
+            const int a = lrp[i], e = lrp[i + 1];
+            for (int k = a; k < e; ++k) {
+                const int j = lci[k];
+                if (j < i) {
+                    const int qu = fu[j]++;
+                    ru.col[qu] = i; ru.sf[qu] = lstart[i] + (k - a); ru.sa[qu] = -1;
                 }
             }
         }
