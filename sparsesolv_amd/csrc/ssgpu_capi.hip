@@ -1824,8 +1824,9 @@ int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const i
     std::vector<int> prp(n + 1, 0), pci((size_t)nnz);
     std::vector<char> pv(vs * (size_t)nnz);
     std::vector<int64_t> vmap((size_t)nnz);
-    std::vector<std::pair<int, int64_t>> row;
-    for (int r = 0; r < n; ++r) {
+    for (int r = 0; r < n; ++r) prp[r + 1] = prp[r] + (row_ptr[perm[r] + 1] - row_ptr[perm[r]]);
+    parallel_for(n, [&](int r) {
+        thread_local std::vector<std::pair<int, int64_t>> row;
         const int i = perm[r];
         row.clear();
         for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) row.emplace_back(iperm[col_idx[k]], k);
@@ -1837,8 +1838,7 @@ int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const i
             std::memcpy(&pv[vs * q], static_cast<const char*>(values) + vs * e.second, vs);
             ++q;
         }
-        prp[r + 1] = q;
-    }
+    });
     ssgpu_matrix h = nullptr;
     const int st = ssgpu_matrix_create(ctx, dtype, n, prp.data(), pci.data(), pv.data(), &h);
     if (st != SSGPU_OK) return st;
