@@ -43,6 +43,27 @@ def algorithmic_bytes(n, nnzA, nnzL, s_a=8, s_f=8, s_v=8):
     return spmv, sweep
 
 
+def copy_rate(reps=5):
+    """HBM rate of a plain 1 GiB device-to-device copy (torch's copy kernel,
+    timed with events on torch's stream): what a streaming kernel reaches on
+    this GPU, read and write bytes both counted."""
+    import torch
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * src.numel() * 8
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"GBps": nbytes / (ms * 1e-3) / 1e9, "ms": ms, "bytes": nbytes}
+
+
 def cpu_baseline(sys_, normB, accel, iters_gpu, solver):
     """Time the compiled reference (or the oracle port if it is absent) on a
     bounded sample: two solves capped at k1 and k2 iterations; setup and
@@ -563,6 +584,9 @@ def main():
     per_it_ms = r.iterate_ms / max(iters, 1)
     print(f"# per-iteration {per_it_ms:.3f} ms; spmv {t_spmv:.3f} ms ({gbs_spmv:.0f} GB/s); "
           f"fwd {t_fwd:.3f} ms bwd {t_bwd:.3f} ms (sweep {gbs_sweep:.0f} GB/s)", file=sys.stderr, flush=True)
+    copy = copy_rate()
+    print(f"# plain 1 GiB device copy: {copy['GBps']:.0f} GB/s; SpMV at {gbs_spmv / copy['GBps']:.2f}x that rate",
+          file=sys.stderr, flush=True)
 
     abmc = abmc_run(a, s, ctx, d_b, normB, code, prec, opts) if a.abmc else None
 
@@ -608,7 +632,9 @@ def main():
                      "traffic": traffic_for(a.n, "sweep"), "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
         "roofline_spmv": {"bound": "hbm", "kernel": "CSR SpMV", "achieved": gbs_spmv, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": gbs_spmv / HBM_PEAK_GBS, "traffic": traffic_for(a.n, "spmv"),
-                          "algorithmic_bytes": b_spmv, "avg_ms": t_spmv},
+                          "algorithmic_bytes": b_spmv, "avg_ms": t_spmv,
+                          "vs_plain_copy": gbs_spmv / copy["GBps"]},
+        "hbm_copy": {**copy, "frac": copy["GBps"] / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "served": served,
         "abmc": abmc,

@@ -720,7 +720,7 @@ struct Solver {
     }
     // Factor the matrix with values av (A or the scaled copy); returns pivot fixes.
     // Sweep launch shape, measured once per matrix on its first factor: the
-    // forward sweep is timed sync-free with 128 and with 1024 waves, and one
+    // forward sweep is timed sync-free with 128, 160 and 1024 waves, and one
     // launch per level when the levels are few; the fastest is kept for the
     // IC sweeps of this matrix.  Every shape gives the same bits.
     template <typename T> void tune_sweeps() {
@@ -735,7 +735,7 @@ struct Solver {
         struct Shape { bool level; int grid; };
         std::vector<Shape> cand;
         if (!fix_mode || !m.sfL.by_level) {
-            for (int g : {128, 1024})
+            for (int g : {128, 160, 1024})
                 if (!fix_grid || g == m.sweep_grid) cand.push_back({false, fix_grid ? m.sweep_grid : g});
         }
         if ((!fix_mode || m.sfL.by_level) && m.L.nlev <= kTuneMaxLevels) cand.push_back({true, m.sweep_grid});
