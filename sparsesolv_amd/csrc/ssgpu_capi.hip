@@ -52,11 +52,10 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t resident = 0;
-    double* h_scal = nullptr;   // pinned scalar read-back
-    DevBuf d_scal;              // device scalar results
+    double* h_scal = nullptr;   // pinned scalar read-back (kScalSlots doubles)
+    DevBuf d_scal;              // device scalar results; the last slot holds the sweep error flag
     DevBuf d_count;
     DevBuf d_tickets;           // sync-free sweep tickets (2 ints)
-    DevBuf d_err;               // sync-free sweep error flag
     std::mutex mu;
     // matrices created on this context and not yet destroyed; a context whose
     // destroy arrives while some are alive is freed with the last of them
@@ -90,6 +89,33 @@ void DevBuf::release() {
     }
     p = nullptr;
     bytes = 0;
+}
+
+// Device scalars of a context: reduction results in the first slots, and in
+// the last slot the flag a sync-free sweep sets when it exceeds its spin
+// limit.  Every scalar read-back copies the whole block, so a solve sees the
+// flag at its next convergence check and stops there.
+constexpr int kScalSlots = 128;
+constexpr int kErrInt = 2 * (kScalSlots - 1);   // the flag, as an int index into the block
+DevBuf& scal_buf(Context* c) {
+    if (c->d_scal.bytes == 0) {
+        c->d_scal.alloc(c, kScalSlots * sizeof(double));
+        launch_fill<int>(c->stream, 2 * kScalSlots, c->d_scal.as<int>(), 0);
+    }
+    return c->d_scal;
+}
+// copy the scalar block to the pinned host copy and synchronize; a set sweep
+// error flag is cleared and reported
+const double* read_scalars(Context* c) {
+    DevBuf& d = scal_buf(c);
+    SSGPU_HIP(hipMemcpyAsync(c->h_scal, d.p, kScalSlots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SSGPU_HIP(hipStreamSynchronize(c->stream));
+    if (reinterpret_cast<const int*>(c->h_scal)[kErrInt] != 0) {
+        launch_fill<int>(c->stream, 1, d.as<int>() + kErrInt, 0);
+        SSGPU_HIP(hipStreamSynchronize(c->stream));
+        throw Error{HIP_ERROR, "a sync-free sweep exceeded its spin limit (a dependency was never produced)"};
+    }
+    return c->h_scal;
 }
 
 template <typename T> void upload(Context* c, DevBuf& d, const std::vector<T>& h) {
@@ -499,11 +525,16 @@ struct Solver {
     Context* c;
     hipStream_t s;
     const ssgpu_options& o;
+    long spin_limit = kSweepSpinLimit;   // polling passes without progress before a sweep wave gives up
     std::vector<double> log;
     int iterations = 0;
     double final_res = 0.0;
     // c: the context whose stream and work space the solve uses
-    Solver(Matrix& mm, Context* cc, const ssgpu_options& oo) : m(mm), c(cc), s(cc->stream), o(oo) {}
+    Solver(Matrix& mm, Context* cc, const ssgpu_options& oo) : m(mm), c(cc), s(cc->stream), o(oo) {
+        // SSGPU_SPIN_LIMIT lowers the sync-free sweeps' spin limit (tests of
+        // the error path force the limit to 0)
+        if (const char* e = std::getenv("SSGPU_SPIN_LIMIT")) spin_limit = std::max(0L, std::atol(e));
+    }
     Solver(Matrix& mm, const ssgpu_options& oo) : Solver(mm, mm.ctx, oo) {}
 
     template <typename T> T* vec(size_t idx) {
@@ -520,11 +551,8 @@ struct Solver {
     }
     // finalize K doubles from nb partials and read them back (synchronizes)
     const double* fetch(int nb, int K) {
-        if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
-        launch_finalize(s, nb, K, c->partial.as<double>(), c->d_scal.as<double>());
-        SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
-        SSGPU_HIP(hipStreamSynchronize(s));
-        return c->h_scal;
+        launch_finalize(s, nb, K, c->partial.as<double>(), scal_buf(c).as<double>());
+        return read_scalars(c);
     }
     template <typename T> double norm(const T* a) {
         launch_norm2<T>(s, m.n, a, partial());
@@ -554,13 +582,7 @@ struct Solver {
         if (c->d_tickets.bytes == 0) c->d_tickets.alloc(c, 64);
         return c->d_tickets.as<int>();
     }
-    int* errflag() {
-        if (c->d_err.bytes == 0) {
-            c->d_err.alloc(c, 64);
-            launch_fill<int>(s, 1, c->d_err.as<int>(), 0);
-        }
-        return c->d_err.as<int>();
-    }
+    int* errflag() { return scal_buf(c).as<int>() + kErrInt; }
     bool sf_ok() const { return m.sfL.kmax > 0 && m.sfUic.kmax > 0; }
     template <typename T, typename F>
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
@@ -571,7 +593,7 @@ struct Solver {
                                       S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux,
                                       dd, x);
         else
-            launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), S.srow,
+            launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), spin_limit, S.srow,
                                   S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd,
                                   m.sweep_idle, x);
     }
@@ -674,17 +696,8 @@ struct Solver {
         m.factor_valid = false;
         m.prep_kind = 2;
     }
-    // sync-free sweep error flag: nonzero if a sweep hit its spin limit (reads back, synchronizes)
-    void check_sweep_error() {
-        if (c->d_err.bytes == 0) return;
-        int e = 0;
-        SSGPU_HIP(hipMemcpyAsync(&e, c->d_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        SSGPU_HIP(hipStreamSynchronize(s));
-        if (e) {
-            launch_fill<int>(s, 1, c->d_err.as<int>(), 0);
-            throw Error{HIP_ERROR, "a sync-free sweep exceeded its spin limit (a dependency was never produced)"};
-        }
-    }
+    // throws if a sync-free sweep hit its spin limit (reads back, synchronizes)
+    void check_sweep_error() { (void)read_scalars(c); }
 
     // --- shifted IC(0) ------------------------------------------------------
     // Update lists of the factorization (once per matrix); false if they would
@@ -1287,16 +1300,15 @@ template <typename T> struct BlockIccg {
     void ensure_scal() {
         if (c->d_bscal.bytes == 0) c->d_bscal.alloc(c, 4096);
         if (!c->h_bscal) SSGPU_HIP(hipHostMalloc(&c->h_bscal, 4096, hipHostMallocDefault));
-        if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
+        (void)scal_buf(c);
     }
     // finalize kb*P partial sums over nb blocks and read them back (synchronizes)
     std::vector<double> fetch(int nb) {
         const int K = kb * P;
         launch_finalize(s, nb, K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
         std::vector<double> out(K);
-        SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
-        SSGPU_HIP(hipStreamSynchronize(s));
-        std::copy(c->h_scal, c->h_scal + K, out.begin());
+        const double* h = read_scalars(c);
+        std::copy(h, h + K, out.begin());
         return out;
     }
     std::vector<T> dots(const std::vector<double>& v) {
@@ -1323,18 +1335,18 @@ template <typename T> struct BlockIccg {
         const int grid = m.block_grid > 0 ? m.block_grid : std::min(m.sweep_grid, std::max(32, 512 / ng));
         const bool f32 = S.o.factor_precision == SSGPU_FACTOR_F32;
         if (f32) {
-            launch_sweep_block<T, F>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), m.sfL.srow,
-                                     m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+            launch_sweep_block<T, F>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit,
+                                     m.sfL.srow, m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
                                      m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r, nullptr, nullptr, tmp);
-            launch_sweep_block<T, F>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(),
+            launch_sweep_block<T, F>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(), S.spin_limit,
                                      m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
                                      m.sfUic.ecol.as<int>(), m.sfUic.eval32.as<F>(), nullptr, nullptr, tmp,
                                      m.diagD.as<T>(), z);
         } else {
-            launch_sweep_block<T, T>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), m.sfL.srow,
-                                     m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+            launch_sweep_block<T, T>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit,
+                                     m.sfL.srow, m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
                                      m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr, nullptr, tmp);
-            launch_sweep_block<T, T>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(),
+            launch_sweep_block<T, T>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(), S.spin_limit,
                                      m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
                                      m.sfUic.ecol.as<int>(), m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
                                      m.diagD.as<T>(), z);
@@ -1370,8 +1382,7 @@ template <typename T> struct BlockIccg {
         {
             const int K = kb;   // norms: one part per column
             launch_finalize(s, reduce_blocks(n), K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
-            SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * K, hipMemcpyDeviceToHost, s));
-            SSGPU_HIP(hipStreamSynchronize(s));
+            (void)read_scalars(c);
         }
         std::vector<double> pure(c->h_scal, c->h_scal + kb);
         constexpr double kSmallAbs = 1.0e-20;
@@ -1410,9 +1421,8 @@ template <typename T> struct BlockIccg {
             std::vector<double> pn;
             {
                 launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
-                SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * kb, hipMemcpyDeviceToHost, s));
-                SSGPU_HIP(hipStreamSynchronize(s));
-                pn.assign(c->h_scal, c->h_scal + kb);
+                const double* h = read_scalars(c);
+                pn.assign(h, h + kb);
             }
             for (int j = 0; j < kb; ++j) {   // Solver::monitor, per column
                 Col& q = col[j];
@@ -1699,7 +1709,7 @@ int ssgpu_context_create(int device, ssgpu_context* out) {
     h->c.device = device;
     SSGPU_HIP(hipSetDevice(device));
     SSGPU_HIP(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
-    SSGPU_HIP(hipHostMalloc((void**)&h->c.h_scal, 64 * sizeof(double), hipHostMallocDefault));
+    SSGPU_HIP(hipHostMalloc((void**)&h->c.h_scal, kScalSlots * sizeof(double), hipHostMallocDefault));
     h->c.owner = h.get();
     *out = h.release();
     return SSGPU_OK;
@@ -1714,7 +1724,6 @@ static void context_free(ssgpu_context ctx) {
     ctx->c.d_scal.release();
     ctx->c.d_count.release();
     ctx->c.d_tickets.release();
-    ctx->c.d_err.release();
     if (ctx->c.h_scal) (void)hipHostFree(ctx->c.h_scal);
     if (ctx->c.h_bscal) (void)hipHostFree(ctx->c.h_bscal);
     (void)hipStreamDestroy(ctx->c.stream);
@@ -2136,7 +2145,7 @@ int ssgpu_dot_device(ssgpu_context ctx, int dtype, int32_t n, const void* d_a, c
     SSGPU_HIP(hipSetDevice(c->device));
     DevBuf part;
     part.alloc(c, sizeof(double) * 2 * kMaxReduceBlocks);
-    if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
+    (void)scal_buf(c);
     const int K = dtype == SSGPU_R64 ? 1 : 2;
     if (dtype == SSGPU_R64) launch_dot<double>(c->stream, n, (const double*)d_a, (const double*)d_b, part.as<double>());
     else launch_dot<zc>(c->stream, n, (const zc*)d_a, (const zc*)d_b, part.as<double>());
@@ -2154,7 +2163,7 @@ int ssgpu_norm_device(ssgpu_context ctx, int dtype, int32_t n, const void* d_a, 
     SSGPU_HIP(hipSetDevice(c->device));
     DevBuf part;
     part.alloc(c, sizeof(double) * kMaxReduceBlocks);
-    if (c->d_scal.bytes == 0) c->d_scal.alloc(c, 64 * sizeof(double));
+    (void)scal_buf(c);
     if (dtype == SSGPU_R64) launch_norm2<double>(c->stream, n, (const double*)d_a, part.as<double>());
     else launch_norm2<zc>(c->stream, n, (const zc*)d_a, part.as<double>());
     launch_finalize(c->stream, reduce_blocks(n), 1, part.as<double>(), c->d_scal.as<double>());

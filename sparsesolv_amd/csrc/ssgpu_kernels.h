@@ -126,12 +126,16 @@ template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d,
 
 // --- sync-free sweeps (ssgpu_sweep.hip) -------------------------------------------
 constexpr int kSfRows = 64;    // rows per chunk = threads per workgroup (one wave)
+// polling passes a sync-free sweep's wave may make without progress before it
+// sets the error flag and leaves (every wave exits, so the grid always drains)
+constexpr long kSweepSpinLimit = 1L << 23;
 // fill a and b (either may be null) with the not-ready sentinel; zero two tickets
 template <typename T> void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets);
 // one sweep over all rows in level order (srow: level position -> row); kmax >=
 // longest row (off-diagonals), <= 64; grid = persistent one-wave workgroups
 template <typename T, typename F>
-void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, long spin_limit,
+                     const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, int max_idle, T* x);
 // the same sweep one level per launch (levptr: host array of nlev + 1 level
@@ -145,6 +149,7 @@ void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int*
 // ncol a multiple of kc; tickets: ncol / kc zeroed counters
 template <typename T, typename F>
 void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
+                        long spin_limit,
                         const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
                         const F* dval, const T* rhs, const T* aux, const T* dd, T* x);
 // --- block solves (ssgpu_block.hip): kb columns, column c at offset c * ld ------------

@@ -34,7 +34,6 @@ namespace {
 
 constexpr unsigned long long kSentinel = 0xFFF40000DEADBEEFull;
 constexpr int kLvlWaves = 4;   // waves (64-row chunks) per workgroup of the per-level sweep
-constexpr long kSpinLimit = 1L << 23;
 
 __device__ inline bool sent(double v) { return (unsigned long long)__double_as_longlong(v) == kSentinel; }
 __device__ inline bool sent(zc v) { return sent(v.re) || sent(v.im); }
@@ -122,7 +121,8 @@ __device__ inline int take_ticket(int* ticket) {
 
 template <typename T, typename F, int MODE, int KMAX>
 __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* __restrict__ ticket,
-                                                      int* __restrict__ err, const int* __restrict__ srow,
+                                                      int* __restrict__ err, long spin_limit,
+                                                      const int* __restrict__ srow,
                                                       const int* __restrict__ slice_off, const int* __restrict__ rlen,
                                                       const int* __restrict__ ecol, const F* __restrict__ eval,
                                                       const F* __restrict__ dval, const T* __restrict__ rhs,
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
                 progress = true;
             }
             if (__all(done)) break;
-            if (++spins > kSpinLimit) {
+            if (++spins > spin_limit) {
                 atomicOr(err, 1);
                 return;
             }
@@ -217,7 +217,8 @@ __global__ __launch_bounds__(kSfRows * WPB) void k_sweep_lvl(int p0, int p1, con
 // result is formed exactly as the one-column sweep forms it.
 template <typename T, typename F, int MODE, int KMAX, int KC>
 __global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int ncol, int* __restrict__ tickets,
-                                                         int* __restrict__ err, const int* __restrict__ srow,
+                                                         int* __restrict__ err, long spin_limit,
+                                                         const int* __restrict__ srow,
                                                          const int* __restrict__ slice_off,
                                                          const int* __restrict__ rlen, const int* __restrict__ ecol,
                                                          const F* __restrict__ eval, const F* __restrict__ dval,
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int
                 progress = true;
             }
             if (__all(done)) break;
-            if (++spins > kSpinLimit) {
+            if (++spins > spin_limit) {
                 atomicOr(err, 1);
                 return;
             }
@@ -370,12 +371,12 @@ inline int grid_for(int64_t n) {
 // register buckets: the row length rounded up (fewer predicated loads per poll
 // pass shortens the pass, and the pass length is part of every level's cost)
 template <typename T, typename F, int MODE>
-void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks, int* ticket, int* err, const int* srow,
-                const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval, const T* rhs,
-                const T* aux, const T* dd, T* x) {
+void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks, int* ticket, int* err,
+                long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
 #define SSGPU_SF(K)                                                                                               \
-    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err, srow, \
-                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, max_idle, x)
+    hipLaunchKernelGGL((k_sweep_sf<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err,        \
+                       spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, max_idle, x)
     if (kmax <= 8) SSGPU_SF(8);
     else if (kmax <= 12) SSGPU_SF(12);
     else if (kmax <= 16) SSGPU_SF(16);
@@ -388,11 +389,11 @@ void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks
 
 template <typename T, typename F, int MODE, int KC>
 void sweep_block_kmax(hipStream_t s, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
-                      const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
-                      const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+                      long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                      const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
 #define SSGPU_SB(K)                                                                                                \
     hipLaunchKernelGGL((k_sweep_block<T, F, MODE, K, KC>), g, dim3(kSfRows), 0, s, n, nchunks, ncol, tickets, err, \
-                       srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+                       spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
     if (kmax <= 8) SSGPU_SB(8);
     else if (kmax <= 12) SSGPU_SB(12);
     else if (kmax <= 16) SSGPU_SB(16);
@@ -405,20 +406,20 @@ void sweep_block_kmax(hipStream_t s, int kmax, dim3 g, int n, int nchunks, int n
 }  // namespace
 
 template <typename T, typename F>
-void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, const int* srow,
-                     const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
-                     const T* rhs, const T* aux, const T* dd, int max_idle, T* x) {
+void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, long spin_limit,
+                     const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
+                     const F* dval, const T* rhs, const T* aux, const T* dd, int max_idle, T* x) {
     const int nchunks = (n + kSfRows - 1) / kSfRows;
     const int g = std::max(1, std::min(grid, nchunks));
     if (mode == FWD_DIV_LAST)
-        sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
-                                       eval, dval, rhs, aux, dd, x);
+        sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow,
+                                       slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
     else if (mode == BWD_IC)
-        sweep_kmax<T, F, BWD_IC>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol, eval,
-                                 dval, rhs, aux, dd, x);
+        sweep_kmax<T, F, BWD_IC>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow,
+                                 slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
     else
-        sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, max_idle, n, nchunks, ticket, err, srow, slice_off, rlen, ecol,
-                                        eval, dval, rhs, aux, dd, x);
+        sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow,
+                                        slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
 template <typename T, typename F, int MODE>
@@ -461,34 +462,34 @@ void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int*
 
 template <typename T, typename F, int KC>
 void sweep_block_kc(hipStream_t s, int mode, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
-                    const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
-                    const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+                    long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                    const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
     if (mode == FWD_DIV_LAST)
-        sweep_block_kmax<T, F, FWD_DIV_LAST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen,
-                                                  ecol, eval, dval, rhs, aux, dd, x);
+        sweep_block_kmax<T, F, FWD_DIV_LAST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
+                                                 slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
     else if (mode == BWD_IC)
-        sweep_block_kmax<T, F, BWD_IC, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol,
-                                            eval, dval, rhs, aux, dd, x);
+        sweep_block_kmax<T, F, BWD_IC, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
+                                           slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
     else
-        sweep_block_kmax<T, F, BWD_DIV_FIRST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen,
-                                                   ecol, eval, dval, rhs, aux, dd, x);
+        sweep_block_kmax<T, F, BWD_DIV_FIRST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
+                                                  slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
 template <typename T, typename F>
 void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
-                        const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
-                        const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+                        long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                        const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
     const int nchunks = (n + kSfRows - 1) / kSfRows;
     const dim3 g((unsigned)std::max(1, std::min(grid, nchunks)), (unsigned)(ncol / kc));
     if (kc == 4 && sizeof(T) == 8)
-        sweep_block_kc<T, F, 4>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
-                                dval, rhs, aux, dd, x);
+        sweep_block_kc<T, F, 4>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
+                                rlen, ecol, eval, dval, rhs, aux, dd, x);
     else if (kc == 2)
-        sweep_block_kc<T, F, 2>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
-                                dval, rhs, aux, dd, x);
+        sweep_block_kc<T, F, 2>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
+                                rlen, ecol, eval, dval, rhs, aux, dd, x);
     else
-        sweep_block_kc<T, F, 1>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, srow, slice_off, rlen, ecol, eval,
-                                dval, rhs, aux, dd, x);
+        sweep_block_kc<T, F, 1>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
+                                rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
 template <typename T>
@@ -503,30 +504,30 @@ void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F
 
 template void launch_sf_prepare<double>(hipStream_t, int, double*, double*, int*);
 template void launch_sf_prepare<zc>(hipStream_t, int, zc*, zc*, int*);
-template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                         const int*, const int*, const double*, const double*, const double*, const double*,
-                                         const double*, int, double*);
-template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                         const int*, const int*, const float*, const float*, const double*, const double*,
-                                         const double*, int, double*);
-template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                         const int*, const int*, const zc*, const zc*, const zc*, const zc*,
-                                         const zc*, int, zc*);
-template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, const int*, const int*,
-                                         const int*, const int*, const fc*, const fc*, const zc*, const zc*,
-                                         const zc*, int, zc*);
-template void launch_sweep_block<double, double>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
-                                            const int*, const int*, const int*, const double*, const double*, const double*,
-                                            const double*, const double*, double*);
-template void launch_sweep_block<double, float>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
-                                            const int*, const int*, const int*, const float*, const float*, const double*,
-                                            const double*, const double*, double*);
-template void launch_sweep_block<zc, zc>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
-                                            const int*, const int*, const int*, const zc*, const zc*, const zc*,
-                                            const zc*, const zc*, zc*);
-template void launch_sweep_block<zc, fc>(hipStream_t, int, int, int, int, int, int, int*, int*, const int*,
-                                            const int*, const int*, const int*, const fc*, const fc*, const zc*,
-                                            const zc*, const zc*, zc*);
+template void launch_sweep_sf<double, double>(hipStream_t, int, int, int, int, int*, int*, long, const int*, const int*,
+                                              const int*, const int*, const double*, const double*, const double*,
+                                              const double*, const double*, int, double*);
+template void launch_sweep_sf<double, float>(hipStream_t, int, int, int, int, int*, int*, long, const int*, const int*,
+                                             const int*, const int*, const float*, const float*, const double*,
+                                             const double*, const double*, int, double*);
+template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int*, long, const int*, const int*,
+                                      const int*, const int*, const zc*, const zc*, const zc*, const zc*, const zc*,
+                                      int, zc*);
+template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, long, const int*, const int*,
+                                      const int*, const int*, const fc*, const fc*, const zc*, const zc*, const zc*,
+                                      int, zc*);
+template void launch_sweep_block<double, double>(hipStream_t, int, int, int, int, int, int, int*, int*, long,
+                                                 const int*, const int*, const int*, const int*, const double*,
+                                                 const double*, const double*, const double*, const double*, double*);
+template void launch_sweep_block<double, float>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
+                                                const int*, const int*, const int*, const float*, const float*,
+                                                const double*, const double*, const double*, double*);
+template void launch_sweep_block<zc, zc>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
+                                         const int*, const int*, const int*, const zc*, const zc*, const zc*, const zc*,
+                                         const zc*, zc*);
+template void launch_sweep_block<zc, fc>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
+                                         const int*, const int*, const int*, const fc*, const fc*, const zc*, const zc*,
+                                         const zc*, zc*);
 template void launch_sweep_levels<double, double>(hipStream_t, int, int, int, const int*, const int*, const int*,
                                              const int*, const int*, const double*, const double*, const double*, const double*,
                                              const double*, double*);
