@@ -81,3 +81,71 @@ def test_full_size_iccg_against_reference(torch_cuda, big):
     assert abs(it_g - it_r) <= max(1, int(0.02 * it_r)), (it_g, it_r)
     np.testing.assert_allclose(log[:6], log_r[:6], rtol=1e-10)
     assert rel <= 1e-8, rel
+
+
+# The complex eddy-current system at its full size: curl-curl + j w sigma M on
+# 64^3 voxels (N = 762,048, nnz = 24,526,920), conductor block sigma = 5.8e7,
+# 50 Hz, COCG at shift 1.1, tol 1e-8.
+@pytest.fixture(scope="module")
+def big_c(torch_cuda, ctx):
+    s = voxel.eddy_current(64, seed=0)
+    print(f"full size complex: N={s.n} nnz={s.nnz}", flush=True)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+    yield s, m
+    m.close()
+
+
+def test_full_size_complex_spmv_and_ic_apply_bit_exact(torch_cuda, ctx, big_c):
+    s, m = big_c
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(s.n) + 1j * rng.standard_normal(s.n)
+    assert np.array_equal(_run(torch_cuda, ctx, m.spmv_device, x), O.spmv(s.row_ptr, s.col_idx, s.values, x))
+    m.ic_factor(1.1)
+    L = O.ic_decomp(s.row_ptr, s.col_idx, s.values, 1.1)
+    U = O.transpose(*L[:3])
+    z = _run(torch_cuda, ctx, m.ic_apply_device, x)
+    assert np.array_equal(z, O.ic_apply(L[:3], U, L[3], x))
+
+
+def test_full_size_cocg_against_reference(torch_cuda, big_c):
+    """Both solutions meet the tolerance, so ||A (x - x_ref)|| / ||b|| is at
+    most twice it (up to residual drift): that bound holds at any size.  x
+    itself is reported and held to 1e-6 relative."""
+    s, m = big_c
+    normB = float(np.linalg.norm(s.rhs))
+    o = capi.default_options(is_save_log=1, accel_factor=1.1)
+    conv, x, log, res = m.solve(capi.ICCG, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
+    A = s.scipy()
+    true_res = float(np.linalg.norm(s.rhs - A @ x) / normB)
+    assert conv
+    assert true_res <= 1.5e-8
+    kw = dict(conv_cri=1e-8, max_ite=20000, accel_factor=1.1, precondition_type=1)
+    if O.reference_available():
+        c_r, x_r, log_r = O.ref_solve("iccg", s.row_ptr, s.col_idx, s.values, s.rhs, normb_mode=0, normB=normB,
+                                      **kw)
+    else:
+        c_r, x_r, log_r = O.solve("iccg", s.row_ptr, s.col_idx, s.values, s.rhs, normB=normB, **kw)
+    it_g, it_r = len(log) - 1, len(log_r) - 1
+    rel = float(np.linalg.norm(x - x_r) / np.linalg.norm(x_r))
+    res_diff = float(np.linalg.norm(A @ (x - x_r)) / normB)
+    print(f"full size COCG: iterations gpu {it_g} reference {it_r}, true residual {true_res:.3e}, "
+          f"x relative difference {rel:.3e}, residual of the difference {res_diff:.3e}", flush=True)
+    assert conv == c_r
+    assert abs(it_g - it_r) <= max(1, int(0.02 * it_r)), (it_g, it_r)
+    np.testing.assert_allclose(log[:6], log_r[:6], rtol=1e-10)
+    assert res_diff <= 3e-8
+    assert rel <= 1e-6
+
+
+def test_full_size_complex64_factor_meets_tolerance(torch_cuda, big_c):
+    """Mixed precision (complex64 factor and sweeps, FP64 elsewhere, FP64
+    true-residual check and refinement): the size-independent property is the
+    true residual, which must meet the tolerance."""
+    s, m = big_c
+    normB = float(np.linalg.norm(s.rhs))
+    o = capi.default_options(accel_factor=1.1, factor_precision=capi.FACTOR_F32)
+    conv, x, log, res = m.solve(capi.ICCG, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
+    true_res = float(np.linalg.norm(s.rhs - s.scipy() @ x) / normB)
+    print(f"full size complex64 factor: iterations {res.iterations}, true residual {true_res:.3e}", flush=True)
+    assert conv
+    assert true_res <= 1.5e-8
