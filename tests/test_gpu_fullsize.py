@@ -149,3 +149,56 @@ def test_full_size_complex64_factor_meets_tolerance(torch_cuda, big_c):
     print(f"full size complex64 factor: iterations {res.iterations}, true residual {true_res:.3e}", flush=True)
     assert conv
     assert true_res <= 1.5e-8
+
+
+def test_full_size_block_solve_matches_solo_solves(torch_cuda, ctx, big):
+    """Four right-hand sides of the n=95 system (the loop coil moved) solved as
+    one block against one cached factor: every column is bit-identical to its
+    solo solve against the same factor, whatever the size."""
+    torch = torch_cuda
+    s, m = big
+    B = np.stack([voxel.curlcurl(95, seed=0, iron_fraction=0.2, coil_seed=k + 1).rhs for k in range(4)])
+    normB = [float(np.linalg.norm(b)) for b in B]
+    o = capi.default_options(accel_factor=1.2)
+    m.factorize(capi.ICCG, 1.2)
+    c2 = capi.Context(0)
+    try:
+        singles = []
+        for b, nb in zip(B, normB):
+            d_b = torch.from_numpy(b.copy()).cuda()
+            d_x = torch.zeros_like(d_b)
+            torch.cuda.synchronize()
+            r = m.solve_factored(c2, capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=20000,
+                                 normB=nb, opts=o)
+            singles.append((d_x.cpu().numpy(), r.iterations, r.converged))
+        d_B = torch.from_numpy(B.copy()).cuda()
+        d_X = torch.zeros_like(d_B)
+        torch.cuda.synchronize()
+        res = m.solve_block(c2, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=1e-8, max_ite=20000, opts=o)
+        X = d_X.cpu().numpy()
+    finally:
+        c2.close()
+    print(f"full size block: iterations {[r.iterations for r in res]}", flush=True)
+    for j, (xs, its, conv) in enumerate(singles):
+        assert conv and res[j].converged
+        assert res[j].iterations == its
+        assert np.array_equal(X[j], xs), j
+
+
+def test_full_size_abmc_meets_tolerance(torch_cuda, ctx):
+    """The ABMC-reordered solve (not a parity mode: IC(0) of P A P^T) of the
+    n=95 system: the true residual of x, returned in the caller's order,
+    meets the tolerance."""
+    s = voxel.curlcurl(95, seed=0, iron_fraction=0.2)
+    m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values, abmc_block=1)
+    try:
+        normB = float(np.linalg.norm(s.rhs))
+        o = capi.default_options(accel_factor=1.2)
+        conv, x, log, res = m.solve(capi.ICCG, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
+    finally:
+        m.close()
+    true_res = float(np.linalg.norm(s.rhs - s.scipy() @ x) / normB)
+    print(f"full size ABMC: {m.ncolors} colors, iterations {res.iterations}, true residual {true_res:.3e}",
+          flush=True)
+    assert conv
+    assert true_res <= 1.5e-8
