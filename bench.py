@@ -64,6 +64,28 @@ def copy_rate(reps=5):
     return {"GBps": nbytes / (ms * 1e-3) / 1e9, "ms": ms, "bytes": nbytes}
 
 
+class _StdoutToStderr:
+    """Send what native code writes to file descriptor 1 (the reference prints
+    "not Convergence" for its capped sample solves) to stderr, so stdout keeps
+    only the JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
 def cpu_baseline(sys_, normB, accel, iters_gpu, solver):
     """Time the compiled reference (or the oracle port if it is absent) on a
     bounded sample: two solves capped at k1 and k2 iterations; setup and
@@ -574,6 +596,8 @@ def main():
     d_y = torch.zeros_like(d_b)
     torch.cuda.synchronize()
     t_spmv = m.time_kernel(0, a.reps, d_b.data_ptr(), d_y.data_ptr())
+    if a.solver == "mrtr_sgs":   # SGS keeps no IC factor; its sweeps have the same shape as the IC sweeps
+        m.time_kernel(4, 1, accel=a.accel, precision=prec)
     t_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
     t_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
     nnzL = (s.nnz + s.n) // 2
@@ -598,7 +622,8 @@ def main():
     cpu = None
     if not a.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(s, normB, a.accel, iters, a.solver)
+        with _StdoutToStderr():
+            cpu = cpu_baseline(s, normB, a.accel, iters, a.solver)
 
     lat_sorted = sorted(lat)
     p50 = statistics.median(lat_sorted)
