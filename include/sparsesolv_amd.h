@@ -31,7 +31,9 @@ typedef enum {
     SSGPU_NO_DIAGONAL = 2,     /* a row of lower(A) does not end with its diagonal */
     SSGPU_BAD_ARG = 3,
     SSGPU_OOM = 4,
-    SSGPU_HIP_ERROR = 5,
+    SSGPU_HIP_ERROR = 5,       /* a HIP call failed, or a sync-free sweep exceeded its spin limit
+                                  (a dependency was never produced); ssgpu_last_error() says which.
+                                  The context and the matrix stay usable. */
     SSGPU_NOT_SQUARE = 6,
     SSGPU_UNSUPPORTED = 7
 } ssgpu_status;
