@@ -33,7 +33,7 @@ typedef enum {
     SSGPU_OOM = 4,
     SSGPU_HIP_ERROR = 5,       /* a HIP call failed, or a sync-free sweep exceeded its spin limit
                                   (a dependency was never produced); ssgpu_last_error() says which.
-                                  The context and the matrix stay usable. */
+                                  After a spin-limit error the context and the matrix stay usable. */
     SSGPU_NOT_SQUARE = 6,
     SSGPU_UNSUPPORTED = 7
 } ssgpu_status;
