@@ -178,6 +178,51 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
                       int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X,
                       ssgpu_result* res);
 
+/* Batched ICCG / COCG over many independent systems (the caller's own
+ * matrices, of any sizes; one matrix may appear several times with different
+ * right-hand sides).  No counterpart in the reference, which solves one system
+ * per call with its Krylov loop on the host (MatSolverICCG.cpp:257-318): here
+ * every phase of the iteration is one launch for all systems of the batch,
+ * the scalar recurrences and the convergence bookkeeping run on the device,
+ * and the iteration is replayed from a hipGraph.  Each system returns the same
+ * x, iteration count and residuals as ssgpu_solve_factored on it alone (same
+ * factor, same options).
+ *
+ * A batch is a session on one context: systems join with ssgpu_batch_add
+ * (the matrix factorized by ssgpu_factorize, ICCG only, no diagonal scaling;
+ * d_b and d_x device vectors of the matrix's n entries that stay valid until
+ * the system's result is polled; d_x is the start vector when
+ * opt->is_init == 0 and receives the solution) and enter the loop at the
+ * next ssgpu_batch_step, which runs `iterations` iterations of every system
+ * still iterating and then finishes those that left the loop (converged,
+ * diverged or at max_ite).  ssgpu_batch_poll returns one finished system at a
+ * time (*got = 0 when none is waiting), with the tag given at add and its
+ * result (no residual log).  Systems may be added between any two steps while
+ * others run (continuous batching).  All matrices of a batch share the value
+ * type (dtype) and the factor precision.  A batch is used by one thread at a
+ * time; it runs on its context's stream.  After a sweep error (SSGPU_HIP_ERROR
+ * from a step) every system of the batch has been dropped with that status;
+ * the batch and the matrices stay usable. */
+typedef struct ssgpu_batch_s* ssgpu_batch;
+int ssgpu_batch_create(ssgpu_context ctx, int dtype, int factor_precision, ssgpu_batch* out);
+int ssgpu_batch_add(ssgpu_batch b, ssgpu_matrix m, const ssgpu_options* opt, double conv_cri, int max_ite,
+                    double normB, const void* d_b, void* d_x, int64_t tag);
+/* Same with b and x in host memory (the reference's pointer overloads,
+ * MatSolverBase.hpp:144-146): both are copied in at the call; x must stay
+ * valid until the result is polled, and receives the solution when the
+ * system finishes. */
+int ssgpu_batch_add_host(ssgpu_batch b, ssgpu_matrix m, const ssgpu_options* opt, double conv_cri, int max_ite,
+                         double normB, const void* b_host, void* x_host, int64_t tag);
+int ssgpu_batch_step(ssgpu_batch b, int iterations, int* running, int* finished);
+int ssgpu_batch_poll(ssgpu_batch b, int* got, int64_t* tag, ssgpu_result* res);
+int ssgpu_batch_destroy(ssgpu_batch b);
+/* One-shot form: nsys systems with shared options, tolerance and iteration
+ * limit; res[i] for system i.  Returns SSGPU_OK when every system converged,
+ * SSGPU_NOT_CONVERGED otherwise. */
+int ssgpu_solve_batch(ssgpu_context ctx, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
+                      double conv_cri, int max_ite, const double* normB, const void* const* d_b, void* const* d_x,
+                      ssgpu_result* res);
+
 /* Kernel-level entry points (device pointers, context stream), used by the
  * parity tests to check each piece against the reference's arithmetic. */
 int ssgpu_spmv_device(ssgpu_matrix m, const void* d_x, void* d_y);

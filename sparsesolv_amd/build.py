@@ -36,11 +36,28 @@ def _run(cmd):
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
+HIP_SOURCES = ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_batch.hip",
+               "ssgpu_order.hip", "ssgpu_capi.hip")
+
+
 def build_hip():
-    srcs = [os.path.join(CSRC, f) for f in ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_order.hip", "ssgpu_capi.hip")]
+    """Each source compiled on its own (in parallel), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(LIB, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    jobs = []
+    for f in HIP_SOURCES:
+        src = os.path.join(CSRC, f)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(objdir, f.replace(".hip", ".o"))
+        jobs.append(([HIPCC, *compile_flags, f"-I{INC}", f"-I{CSRC}", "-c", src, "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=min(len(jobs), max(1, min(8, os.cpu_count() or 1)))) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
     out = os.path.join(LIB, "libsparsesolv_amd.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", *srcs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *[o for _, o in jobs], "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -80,6 +80,13 @@ def lib():
         L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
         L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_solve_block.argtypes = [V, V, P(Options), D, I, I, V, V, V, P(Result)]
+        L.ssgpu_batch_create.argtypes = [V, I, I, P(V)]
+        L.ssgpu_batch_add.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
+        L.ssgpu_batch_add_host.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
+        L.ssgpu_batch_step.argtypes = [V, I, P(I), P(I)]
+        L.ssgpu_batch_poll.argtypes = [V, P(I), P(ctypes.c_int64), P(Result)]
+        L.ssgpu_batch_destroy.argtypes = [V]
+        L.ssgpu_solve_batch.argtypes = [V, I, P(V), P(Options), D, I, V, P(V), P(V), P(Result)]
         L.ssgpu_spmv_device.argtypes = [V, V, V]
         L.ssgpu_ic_factor_host.argtypes = [V, D, V, V, V, V, P(I)]
         L.ssgpu_ic_apply_device.argtypes = [V, V, V]
@@ -278,6 +285,71 @@ class Matrix:
 
     def upper_sweep_device(self, d_r: int, d_z: int):
         check(lib().ssgpu_upper_sweep_device(self.h, d_r, d_z))
+
+
+class Batch:
+    """A batched ICCG / COCG session on one context (ssgpu_batch_*): systems
+    join with add(), advance with step(), and leave through poll()."""
+
+    def __init__(self, ctx: Context, complex_: bool = False, precision=FACTOR_F64):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().ssgpu_batch_create(ctx.h, C128 if complex_ else R64, precision, ctypes.byref(self.h)))
+
+    def add(self, m: "Matrix", d_b: int, d_x: int, tag: int, *, normB, conv_cri=1e-8, max_ite=10000, opts=None):
+        o = opts or default_options()
+        check(lib().ssgpu_batch_add(self.h, m.h, ctypes.byref(o), conv_cri, max_ite, normB, d_b, d_x, int(tag)))
+
+    def add_host(self, m: "Matrix", b: np.ndarray, x: np.ndarray, tag: int, *, normB, conv_cri=1e-8,
+                 max_ite=10000, opts=None):
+        """b and x host arrays of the matrix's dtype (C-contiguous); x is the start vector when
+        opts.is_init == 0 and receives the solution; keep it alive until its result is polled."""
+        o = opts or default_options()
+        assert b.dtype == m.dtype and x.dtype == m.dtype and b.flags.c_contiguous and x.flags.c_contiguous
+        assert b.shape == (m.n,) and x.shape == (m.n,)
+        check(lib().ssgpu_batch_add_host(self.h, m.h, ctypes.byref(o), conv_cri, max_ite, normB, b.ctypes.data,
+                                         x.ctypes.data, int(tag)))
+
+    def step(self, iterations: int):
+        """Run that many iterations; returns (systems in the batch, results waiting)."""
+        running, finished = ctypes.c_int(), ctypes.c_int()
+        check(lib().ssgpu_batch_step(self.h, int(iterations), ctypes.byref(running), ctypes.byref(finished)))
+        return running.value, finished.value
+
+    def poll(self):
+        """[(tag, Result)] of every finished system waiting."""
+        out = []
+        while True:
+            got, tag, res = ctypes.c_int(), ctypes.c_int64(), Result()
+            check(lib().ssgpu_batch_poll(self.h, ctypes.byref(got), ctypes.byref(tag), ctypes.byref(res)))
+            if not got.value:
+                return out
+            out.append((tag.value, res))
+
+    def close(self):
+        if self.h:
+            lib().ssgpu_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batch(ctx: Context, mats, d_bs, d_xs, normBs, *, conv_cri=1e-8, max_ite=10000, opts=None):
+    """One-shot batched ICCG / COCG (ssgpu_solve_batch); one Result per system."""
+    k = len(mats)
+    o = opts or default_options()
+    M = (ctypes.c_void_p * k)(*[m.h.value for m in mats])
+    Bp = (ctypes.c_void_p * k)(*d_bs)
+    Xp = (ctypes.c_void_p * k)(*d_xs)
+    nb = np.ascontiguousarray(normBs, dtype=np.float64)
+    res = (Result * k)()
+    st = lib().ssgpu_solve_batch(ctx.h, k, M, ctypes.byref(o), conv_cri, max_ite, nb.ctypes.data, Bp, Xp, res)
+    check(st, allow=(OK, NOT_CONVERGED))
+    return list(res)
 
 
 def dot_device(ctx: Context, complex_: bool, n: int, d_a: int, d_b: int):

@@ -1,0 +1,636 @@
+// Batched ICCG / COCG: many independent systems (uneven sizes, any mix of
+// matrices, several right-hand sides of one matrix) advanced together, one
+// launch per phase of the iteration for all of them.
+//
+// The reference solves one system per call and keeps the Krylov loop on the
+// host (MatSolverICCG.cpp:257-318).  Here the loop's scalars live on the
+// device, one BatchState per system: the finalize kernels evaluate alpha,
+// beta, the residual norm and the reference's convergence bookkeeping for
+// every system at once, and a system that converges, diverges or reaches its
+// iteration limit turns every later launch into a no-op for itself.  The nine
+// launches of one iteration are captured once into a hipGraph and replayed;
+// the host looks at the batch only between replays (one small read-back),
+// where finished systems leave and newly added ones join (continuous
+// batching).  The sweeps of all systems run in one persistent launch
+// (k_batch_sweep), so the dependency chains of the systems advance side by
+// side instead of one after another.
+//
+// Every per-system operation is the one-system kernel's, on the same blocks
+// and reduction tree, so each system's x, iteration count and residuals are
+// bit-identical to its solo solve against the same factor
+// (ssgpu_solve_factored).
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <string>
+
+#include "ssgpu_host.h"
+
+namespace ssgpu {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum SlotState { SLOT_FREE = 0, SLOT_NEW = 1, SLOT_RUN = 2 };
+
+struct Slot {
+    int state = SLOT_FREE;
+    Matrix* m = nullptr;
+    int64_t tag = 0;
+    ssgpu_options opt{};
+    double conv_cri = 0.0, normB = 0.0;
+    int max_ite = 0;
+    const void* user_b = nullptr;
+    void* user_x = nullptr;
+    bool own_bx = false;   // b and x live in private copies (ABMC-ordered matrix, or host arrays)
+    bool host_io = false;  // b and x are host arrays: copied in at add, x copied out at finish
+    std::unique_ptr<DevBuf> vecs, part;
+    Clock::time_point t0;
+};
+
+struct Batch {
+    Context* c = nullptr;
+    int dtype = SSGPU_R64;
+    int precision = SSGPU_FACTOR_F64;
+    long spin_limit = kSweepSpinLimit;
+    int sweep_grid = 1024;
+    bool use_graph = true;
+    std::vector<Slot> slots;
+    std::vector<BatchSys> h_sys;
+    std::vector<BatchState> h_state;
+    DevBuf d_sys, d_state, d_items_mv, d_items_red, d_fin, d_ford, d_bord, d_flags;
+    int n_mv = 0, n_red = 0, n_fin = 0, n_ford = 0, n_bord = 0, kmax_f = 1, kmax_b = 1;
+    int cap = 0;
+    bool dirty = false;
+    bool has_new = false;
+    int* h_flags = nullptr;   // pinned: [0] active count, [1] sweep error flag
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::deque<std::pair<int64_t, ssgpu_result>> done;
+    int64_t iterations_run = 0;
+
+    int* d_active() { return d_flags.as<int>(); }
+    int* d_err() { return d_flags.as<int>() + 1; }
+    int* d_tickets() { return d_flags.as<int>() + 2; }
+
+    ~Batch() { drop_graph(); if (h_flags) (void)hipHostFree(h_flags); }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+template <typename T> constexpr int parts() { return sizeof(T) / sizeof(double); }
+
+// chunk levels of a sweep structure: the level of each chunk's first row
+std::vector<int> chunk_levels(const LevelCsr& lv, int n) {
+    const int nch = (n + kSfRows - 1) / kSfRows;
+    std::vector<int> out(nch);
+    int l = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int p = c * kSfRows;
+        while (l + 1 < (int)lv.levptr.size() - 1 && lv.levptr[l + 1] <= p) ++l;
+        out[c] = l;
+    }
+    return out;
+}
+
+// ticket order of one sweep direction: every system's chunks in its own
+// level order, systems interleaved by chunk level (counting sort on the
+// level, stable in (slot, chunk))
+std::vector<int2> merged_order(const std::vector<std::pair<int, const std::vector<int>*>>& sys_levels) {
+    int maxl = 0;
+    size_t total = 0;
+    for (const auto& sl : sys_levels) {
+        total += sl.second->size();
+        if (!sl.second->empty()) maxl = std::max(maxl, sl.second->back());
+    }
+    std::vector<int> cnt(maxl + 2, 0);
+    for (const auto& sl : sys_levels)
+        for (int l : *sl.second) cnt[l + 1]++;
+    for (int l = 0; l <= maxl; ++l) cnt[l + 1] += cnt[l];
+    std::vector<int2> out(total);
+    for (const auto& sl : sys_levels) {
+        const std::vector<int>& lv = *sl.second;
+        for (int ch = 0; ch < (int)lv.size(); ++ch) out[cnt[lv[ch]]++] = make_int2(sl.first, ch);
+    }
+    return out;
+}
+
+template <typename T> void upload_vec(Context* c, DevBuf& d, const std::vector<T>& h) {
+    const size_t need = sizeof(T) * std::max<size_t>(h.size(), 1);
+    if (d.bytes < need) d.alloc(c, need + need / 2);
+    if (!h.empty()) SSGPU_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+}
+
+template <typename T> struct Runner {
+    using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
+    Batch& B;
+    hipStream_t s;
+    explicit Runner(Batch& b) : B(b), s(b.c->stream) {}
+
+    const BatchSys* sys() { return B.d_sys.as<BatchSys>(); }
+    BatchState* st() { return B.d_state.as<BatchState>(); }
+    const int2* mv() { return B.d_items_mv.as<int2>(); }
+    const int2* red() { return B.d_items_red.as<int2>(); }
+    const int* fin() { return B.d_fin.as<int>(); }
+
+    void vec(int op, int want, int a, int b = -1, int c = -1) {
+        launch_batch_vec<T>(s, op, B.n_red, red(), sys(), st(), want, a, b, c);
+    }
+    void finalize(int phase, int want) {
+        launch_batch_finalize<T>(s, phase, B.n_fin, fin(), sys(), st(), want, B.d_active(), B.d_tickets());
+    }
+    void sweeps(int want, int in, int mid, int out) {
+        int* tk = B.d_tickets();
+        if (B.precision == SSGPU_FACTOR_F32) {
+            launch_batch_sweep<T, F>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
+                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2);
+            launch_batch_sweep<T, F>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
+                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out,
+                                     2);
+        } else {
+            launch_batch_sweep<T, T>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
+                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2);
+            launch_batch_sweep<T, T>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
+                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out,
+                                     2);
+        }
+    }
+
+    // the prologue of the systems added since the last step: ||r0|| and the
+    // reference's early exits, z0 = M^-1 r0, p = z0, (r, z)
+    void prologue() {
+        vec(BO_NORM2, BS_PENDING, BV_R);
+        finalize(BP_PROLOGUE, BS_PENDING);
+        vec(BO_SENT, BS_PRIMED, BV_T, BV_P);
+        sweeps(BS_PRIMED, BV_R, BV_T, BV_P);
+        vec(BO_COPY, BS_PRIMED, BV_P, BV_Z);
+        vec(BO_DOT, BS_PRIMED, BV_R, BV_Z);
+        finalize(BP_RZ0, BS_PRIMED);
+    }
+    // one iteration of MatSolverICCG.cpp:257-318 for every active system
+    void iteration() {
+        launch_batch_spmv<T>(s, B.n_mv, mv(), sys(), st(), BS_ACTIVE, BV_P, BV_Q, 1);
+        finalize(BP_ALPHA, BS_ACTIVE);
+        vec(BO_CG_UPDATE, BS_ACTIVE, 0);
+        finalize(BP_MONITOR, BS_ACTIVE);
+        sweeps(BS_ACTIVE, BV_R, BV_T, BV_Z);
+        vec(BO_DOT, BS_ACTIVE, BV_R, BV_Z);
+        finalize(BP_BETA, BS_ACTIVE);
+        vec(BO_PUPDATE, BS_ACTIVE, 0);
+    }
+
+    void capture() {
+        B.drop_graph();
+        if (!B.use_graph || B.n_fin == 0) return;
+        SSGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iteration();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            throw Error{HIP_ERROR, std::string("graph capture of the batched iteration failed: ") + hipGetErrorString(e)};
+        }
+        B.graph = g;
+        SSGPU_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    }
+
+    // rebuild the device tables for the systems in the batch, run the
+    // prologue of the new ones and capture the iteration
+    void rebuild() {
+        Context* c = B.c;
+        const int ns = (int)B.slots.size();
+        if (B.cap < ns) {   // grow the descriptor and state arrays (states of running systems kept)
+            const int ncap = std::max(ns, 2 * B.cap);
+            DevBuf nsys, nst;
+            nsys.alloc(c, sizeof(BatchSys) * ncap);
+            nst.alloc(c, sizeof(BatchState) * ncap);
+            if (B.cap > 0)
+                SSGPU_HIP(hipMemcpyAsync(nst.p, B.d_state.p, sizeof(BatchState) * B.cap, hipMemcpyDeviceToDevice, s));
+            std::swap(B.d_sys.p, nsys.p);
+            std::swap(B.d_sys.bytes, nsys.bytes);
+            std::swap(B.d_state.p, nst.p);
+            std::swap(B.d_state.bytes, nst.bytes);
+            B.d_sys.ctx = B.d_state.ctx = c;
+            B.cap = ncap;
+            SSGPU_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<int2> imv, ired;
+        std::vector<int> fin;
+        std::vector<std::pair<int, const std::vector<int>*>> fl, bl;
+        std::vector<std::pair<Matrix*, std::pair<std::vector<int>, std::vector<int>>>> lev_cache;
+        auto levels_of = [&](Matrix* m) -> std::pair<std::vector<int>, std::vector<int>>& {
+            for (auto& e : lev_cache)
+                if (e.first == m) return e.second;
+            lev_cache.emplace_back(m, std::make_pair(chunk_levels(*m->sfL.lv, m->n), chunk_levels(*m->sfUic.lv, m->n)));
+            return lev_cache.back().second;
+        };
+        lev_cache.reserve(ns);
+        int kf = 1, kb = 1, active = 0;
+        for (int i = 0; i < ns; ++i) {
+            Slot& q = B.slots[i];
+            if (q.state == SLOT_FREE) continue;
+            const BatchSys& d = B.h_sys[i];
+            for (int b = 0; b < d.nblk; ++b) imv.push_back(make_int2(i, b));
+            for (int b = 0; b < d.rblk; ++b) ired.push_back(make_int2(i, b));
+            fin.push_back(i);
+            auto& lv = levels_of(q.m);
+            fl.emplace_back(i, &lv.first);
+            bl.emplace_back(i, &lv.second);
+            kf = std::max(kf, q.m->sfL.kmax);
+            kb = std::max(kb, q.m->sfUic.kmax);
+            ++active;
+        }
+        const std::vector<int2> ford = merged_order(fl), bord = merged_order(bl);
+        upload_vec(c, B.d_items_mv, imv);
+        upload_vec(c, B.d_items_red, ired);
+        upload_vec(c, B.d_fin, fin);
+        upload_vec(c, B.d_ford, ford);
+        upload_vec(c, B.d_bord, bord);
+        SSGPU_HIP(hipMemcpyAsync(B.d_sys.p, B.h_sys.data(), sizeof(BatchSys) * ns, hipMemcpyHostToDevice, s));
+        for (int i = 0; i < ns; ++i)
+            if (B.slots[i].state == SLOT_NEW)
+                SSGPU_HIP(hipMemcpyAsync(B.d_state.as<BatchState>() + i, &B.h_state[i], sizeof(BatchState),
+                                         hipMemcpyHostToDevice, s));
+        B.h_flags[0] = active;
+        B.h_flags[1] = 0;
+        SSGPU_HIP(hipMemcpyAsync(B.d_active(), B.h_flags, sizeof(int), hipMemcpyHostToDevice, s));
+        B.n_mv = (int)imv.size();
+        B.n_red = (int)ired.size();
+        B.n_fin = (int)fin.size();
+        B.n_ford = (int)ford.size();
+        B.n_bord = (int)bord.size();
+        B.kmax_f = kf;
+        B.kmax_b = kb;
+        SSGPU_HIP(hipStreamSynchronize(s));   // the host tables above are temporaries
+        if (B.has_new) prologue();
+        for (auto& q : B.slots)
+            if (q.state == SLOT_NEW) q.state = SLOT_RUN;
+        B.has_new = false;
+        capture();
+        B.dirty = false;
+    }
+
+    // a system that left the loop: best-iterate restore, x back to the
+    // caller's order, true residual, result
+    void finish(int i, const BatchState& q) {
+        Slot& sl = B.slots[i];
+        Matrix& m = *sl.m;
+        const BatchSys& d = B.h_sys[i];
+        T* x = static_cast<T*>(d.vec[BV_X]);
+        const T* b = static_cast<const T*>(d.vec[BV_B]);
+        const bool conv = q.status == BS_CONVERGED || q.status == BS_EARLY;
+        // MatSolverICCG.cpp:315-317: on failure with save-best, the best iterate
+        // (the current x when the last check had just saved it)
+        if (!conv && sl.opt.is_save_best && !q.copy_best)
+            launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_BEST]), x);
+        int refinements = 0, refine_its = 0;
+        const double true_res = batch_finish<T>(B.c, m, sl.opt, conv, sl.conv_cri, sl.max_ite, sl.normB, b, x,
+                                                refinements, refine_its);
+        if (sl.own_bx) {   // x back in the caller's order, to the caller's device or host array
+            T* dst = sl.host_io ? static_cast<T*>(d.vec[BV_T]) : static_cast<T*>(sl.user_x);
+            if (m.ordered) launch_gather<T, T>(s, m.n, m.iperm.as<int>(), x, dst);
+            else if (!sl.host_io) launch_copy<T>(s, m.n, x, dst);
+            if (sl.host_io)
+                SSGPU_HIP(hipMemcpyAsync(sl.user_x, m.ordered ? dst : x, sizeof(T) * m.n, hipMemcpyDeviceToHost, s));
+        }
+        SSGPU_HIP(hipStreamSynchronize(s));
+        ssgpu_result r{};
+        r.converged = conv ? 1 : 0;
+        r.status = conv ? OK : NOT_CONVERGED;
+        r.iterations = q.iterations;
+        r.log_len = 0;
+        r.final_residual = q.final_res;
+        r.accel_used = m.factor_accel;
+        r.pivot_fixes = m.factor_pivot_fixes;
+        r.setup_ms = 0.0;
+        r.iterate_ms = std::chrono::duration<double, std::milli>(Clock::now() - sl.t0).count();
+        r.true_residual = true_res;
+        r.refinements = refinements;
+        r.refine_iterations = refine_its;
+        B.done.emplace_back(sl.tag, r);
+        sl.state = SLOT_FREE;
+        sl.m = nullptr;
+        B.dirty = true;
+    }
+
+    // run `iters` iterations (fewer if every system finishes), then finish
+    // the systems that left the loop
+    void step(int iters) {
+        if (B.dirty) rebuild();
+        if (B.n_fin == 0) return;
+        for (int k = 0; k < iters; ++k) {
+            if (B.exec) SSGPU_HIP(hipGraphLaunch(B.exec, s));
+            else iteration();
+        }
+        B.iterations_run += iters;
+        SSGPU_HIP(hipMemcpyAsync(B.h_flags, B.d_flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        SSGPU_HIP(hipMemcpyAsync(B.h_state.data(), B.d_state.p, sizeof(BatchState) * B.slots.size(),
+                                 hipMemcpyDeviceToHost, s));
+        SSGPU_HIP(hipStreamSynchronize(s));
+        if (B.h_flags[1] != 0) {
+            // a sweep gave up: every system of the batch is dropped, the flag cleared
+            launch_fill<int>(s, 1, B.d_err(), 0);
+            for (auto& q : B.slots) {
+                if (q.state == SLOT_FREE) continue;
+                ssgpu_result r{};
+                r.status = HIP_ERROR;
+                B.done.emplace_back(q.tag, r);
+                q.state = SLOT_FREE;
+                q.m = nullptr;
+            }
+            B.dirty = true;
+            SSGPU_HIP(hipStreamSynchronize(s));
+            throw Error{HIP_ERROR, "a batched sweep exceeded its spin limit (a dependency was never produced)"};
+        }
+        for (int i = 0; i < (int)B.slots.size(); ++i) {
+            if (B.slots[i].state != SLOT_RUN) continue;
+            const int stt = B.h_state[i].status;
+            if (stt == BS_CONVERGED || stt == BS_STOPPED || stt == BS_EARLY) finish(i, B.h_state[i]);
+        }
+    }
+
+    // queue one system; it joins the loop at the next step
+    void add(Matrix& m, const ssgpu_options& o, double conv_cri, int max_ite, double normB, const void* ub, void* ux,
+             int64_t tag, bool host_io) {
+        Context* c = B.c;
+        int i = 0;
+        while (i < (int)B.slots.size() && B.slots[i].state != SLOT_FREE) ++i;
+        if (i == (int)B.slots.size()) {
+            B.slots.emplace_back();
+            B.h_sys.emplace_back();
+            B.h_state.emplace_back();
+        }
+        Slot& sl = B.slots[i];
+        const int n = m.n;
+        const size_t vbytes = sizeof(T) * (size_t)n;
+        if (!sl.vecs) sl.vecs.reset(new DevBuf());
+        if (!sl.part) sl.part.reset(new DevBuf());
+        if (sl.vecs->bytes < vbytes * kBatchVecs) sl.vecs->alloc(c, vbytes * kBatchVecs);
+        const int rblk = reduce_blocks(n);
+        const size_t pbytes = sizeof(double) * 2 * (size_t)std::max(m.nblk, rblk);
+        if (sl.part->bytes < pbytes) sl.part->alloc(c, pbytes);
+        BatchSys d{};
+        d.n = n;
+        d.nblk = m.nblk;
+        d.rblk = rblk;
+        d.blk = m.blk.as<int>();
+        d.rp = m.rp.as<int>();
+        d.ci = m.ci.as<int>();
+        d.val = m.val.p;
+        const bool f32 = B.precision == SSGPU_FACTOR_F32;
+        d.f_srow = m.sfL.srow;
+        d.f_soff = m.sfL.slice_off.as<int>();
+        d.f_rlen = m.sfL.rlen.as<int>();
+        d.f_ecol = m.sfL.ecol.as<int>();
+        d.f_eval = f32 ? m.sfL.eval32.p : m.sfL.eval.p;
+        d.f_dval = f32 ? m.sfL.dval32.p : m.sfL.dval.p;
+        d.b_srow = m.sfUic.srow;
+        d.b_soff = m.sfUic.slice_off.as<int>();
+        d.b_rlen = m.sfUic.rlen.as<int>();
+        d.b_ecol = m.sfUic.ecol.as<int>();
+        d.b_eval = f32 ? m.sfUic.eval32.p : m.sfUic.eval.p;
+        d.diagD = m.diagD.p;
+        char* base = static_cast<char*>(sl.vecs->p);
+        for (int k = 0; k < kBatchVecs; ++k) d.vec[k] = base + vbytes * k;
+        d.part = sl.part->as<double>();
+        sl.own_bx = m.ordered || host_io;
+        sl.host_io = host_io;
+        if (sl.own_bx) {
+            // private b and x in the matrix's order (PermutedIO in the one-system
+            // path); host arrays come in through two scratch vectors
+            const T* sb = static_cast<const T*>(ub);
+            const T* sx = static_cast<const T*>(ux);
+            if (host_io) {
+                T* tb = static_cast<T*>(m.ordered ? d.vec[BV_T] : d.vec[BV_B]);
+                T* tx = static_cast<T*>(m.ordered ? d.vec[BV_Q] : d.vec[BV_X]);
+                SSGPU_HIP(hipMemcpyAsync(tb, ub, vbytes, hipMemcpyHostToDevice, s));
+                SSGPU_HIP(hipMemcpyAsync(tx, ux, vbytes, hipMemcpyHostToDevice, s));
+                sb = tb;
+                sx = tx;
+            }
+            if (m.ordered) {
+                launch_gather<T, T>(s, n, m.perm.as<int>(), sb, static_cast<T*>(d.vec[BV_B]));
+                launch_gather<T, T>(s, n, m.perm.as<int>(), sx, static_cast<T*>(d.vec[BV_X]));
+            }
+        } else {
+            d.vec[BV_B] = const_cast<void*>(ub);
+            d.vec[BV_X] = ux;
+        }
+        // the first lines of iccg_main, on this system alone
+        T* x = static_cast<T*>(d.vec[BV_X]);
+        const T* b = static_cast<const T*>(d.vec[BV_B]);
+        T* r = static_cast<T*>(d.vec[BV_R]);
+        T* q = static_cast<T*>(d.vec[BV_Q]);
+        if (o.is_init) launch_fill<T>(s, n, x, T{});
+        if (o.is_save_best) launch_fill<T>(s, n, static_cast<T*>(d.vec[BV_BEST]), T{});
+        if (!o.is_init) {
+            launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), x, q, nullptr,
+                           nullptr, m.spmv_stage_kb);
+            launch_residual<T>(s, n, b, q, r);
+        } else {
+            launch_residual<T>(s, n, b, nullptr, r);
+        }
+        BatchState st{};
+        st.normB = normB;
+        st.normalizer = 1.0;
+        st.best = 1.0e+6;
+        st.conv_cri = conv_cri;
+        st.bad_div_val = o.bad_div_val;
+        st.normalize_const = o.conv_normalize_const;
+        st.max_ite = max_ite;
+        st.status = BS_PENDING;
+        st.normalize_type = o.conv_normalize_type;
+        st.save_best = o.is_save_best;
+        st.diverge_type = o.diverge_judge_type;
+        st.bad_div_thres = o.bad_div_count_thres;
+        B.h_sys[i] = d;
+        B.h_state[i] = st;
+        sl.state = SLOT_NEW;
+        sl.m = &m;
+        sl.tag = tag;
+        sl.opt = o;
+        sl.conv_cri = conv_cri;
+        sl.max_ite = max_ite;
+        sl.normB = normB;
+        sl.user_b = ub;
+        sl.user_x = ux;
+        sl.t0 = Clock::now();
+        B.dirty = true;
+        B.has_new = true;
+    }
+};
+
+}  // namespace
+
+}  // namespace ssgpu
+
+using namespace ssgpu;
+
+struct ssgpu_batch_s {
+    Batch b;
+};
+
+namespace {
+
+void check_batch_matrix(const Batch& B, const Matrix& m) {
+    if (m.ctx->device != B.c->device) throw Error{BAD_ARG, "matrix and batch are on different devices"};
+    if (m.dtype != B.dtype) throw Error{BAD_ARG, "matrix value type differs from the batch's"};
+    if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
+    if (B.precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+        throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
+    if (m.sfL.kmax <= 0 || m.sfUic.kmax <= 0)
+        throw Error{UNSUPPORTED, "batched solves need rows of at most 64 off-diagonal entries"};
+}
+
+template <typename F> int with_runner(Batch& B, F&& f) {
+    if (B.dtype == SSGPU_R64) {
+        Runner<double> r(B);
+        f(r);
+    } else {
+        Runner<zc> r(B);
+        f(r);
+    }
+    return SSGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssgpu_batch_create(ssgpu_context ctx, int dtype, int factor_precision, ssgpu_batch* out) {
+    if (!ctx || !out || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
+    if (factor_precision != SSGPU_FACTOR_F64 && factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    std::unique_ptr<ssgpu_batch_s> h(new ssgpu_batch_s());
+    Batch& B = h->b;
+    B.c = &ctx->c;
+    B.dtype = dtype;
+    B.precision = factor_precision;
+    if (const char* e = std::getenv("SSGPU_SPIN_LIMIT")) B.spin_limit = std::max(0L, std::atol(e));
+    if (const char* e = std::getenv("SSGPU_BATCH_GRAPH")) B.use_graph = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SSGPU_BATCH_GRID")) B.sweep_grid = std::max(1, std::atoi(e));
+    SSGPU_HIP(hipSetDevice(B.c->device));
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    B.d_flags.alloc(B.c, 64);
+    launch_fill<int>(B.c->stream, 16, B.d_flags.as<int>(), 0);
+    SSGPU_HIP(hipHostMalloc((void**)&B.h_flags, 64, hipHostMallocDefault));
+    SSGPU_HIP(hipStreamSynchronize(B.c->stream));
+    *out = h.release();
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_batch_add(ssgpu_batch bh, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
+                    double normB, const void* d_b, void* d_x, int64_t tag) {
+    if (!bh || !mh || !opt || !d_b || !d_x) return SSGPU_BAD_ARG;
+    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    SSGPU_GUARD_BEGIN
+    Batch& B = bh->b;
+    Matrix& m = mh->m;
+    check_batch_matrix(B, m);
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    SSGPU_HIP(hipSetDevice(B.c->device));
+    return with_runner(B, [&](auto& r) { r.add(m, *opt, conv_cri, max_ite, normB, d_b, d_x, tag, false); });
+    SSGPU_GUARD_END
+}
+
+int ssgpu_batch_add_host(ssgpu_batch bh, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
+                         double normB, const void* b, void* x, int64_t tag) {
+    if (!bh || !mh || !opt || !b || !x) return SSGPU_BAD_ARG;
+    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    SSGPU_GUARD_BEGIN
+    Batch& B = bh->b;
+    Matrix& m = mh->m;
+    check_batch_matrix(B, m);
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    SSGPU_HIP(hipSetDevice(B.c->device));
+    return with_runner(B, [&](auto& r) { r.add(m, *opt, conv_cri, max_ite, normB, b, x, tag, true); });
+    SSGPU_GUARD_END
+}
+
+int ssgpu_batch_step(ssgpu_batch bh, int iterations, int* running, int* finished) {
+    if (!bh || iterations < 0) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    Batch& B = bh->b;
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    SSGPU_HIP(hipSetDevice(B.c->device));
+    with_runner(B, [&](auto& r) { r.step(iterations); });
+    if (running) {
+        int k = 0;
+        for (const auto& q : B.slots) k += q.state != SLOT_FREE;
+        *running = k;
+    }
+    if (finished) *finished = (int)B.done.size();
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_batch_poll(ssgpu_batch bh, int* got, int64_t* tag, ssgpu_result* res) {
+    if (!bh || !got) return SSGPU_BAD_ARG;
+    Batch& B = bh->b;
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    *got = 0;
+    if (B.done.empty()) return SSGPU_OK;
+    if (tag) *tag = B.done.front().first;
+    if (res) *res = B.done.front().second;
+    B.done.pop_front();
+    *got = 1;
+    return SSGPU_OK;
+}
+
+int ssgpu_batch_destroy(ssgpu_batch bh) {
+    if (!bh) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    {
+        std::lock_guard<std::mutex> lk(bh->b.c->mu);
+        (void)hipSetDevice(bh->b.c->device);
+        (void)hipStreamSynchronize(bh->b.c->stream);
+    }
+    delete bh;
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_solve_batch(ssgpu_context ctx, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
+                      double conv_cri, int max_ite, const double* normB, const void* const* d_b, void* const* d_x,
+                      ssgpu_result* res) {
+    if (!ctx || nsys < 1 || !mats || !opt || !normB || !d_b || !d_x || !res) return SSGPU_BAD_ARG;
+    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    for (int i = 0; i < nsys; ++i)
+        if (!mats[i] || !d_b[i] || !d_x[i]) return SSGPU_BAD_ARG;
+    const int dtype = mats[0]->m.dtype;
+    ssgpu_batch bh = nullptr;
+    int st = ssgpu_batch_create(ctx, dtype, opt->factor_precision, &bh);
+    if (st != SSGPU_OK) return st;
+    for (int i = 0; i < nsys && st == SSGPU_OK; ++i)
+        st = ssgpu_batch_add(bh, mats[i], opt, conv_cri, max_ite, normB[i], d_b[i], d_x[i], i);
+    int running = nsys;
+    int chunk = 8;
+    if (const char* e = std::getenv("SSGPU_BATCH_CHECK")) chunk = std::max(1, std::atoi(e));
+    while (st == SSGPU_OK && running > 0) st = ssgpu_batch_step(bh, chunk, &running, nullptr);
+    int all_conv = 1;
+    if (st == SSGPU_OK) {
+        int got = 1;
+        int64_t tag = 0;
+        ssgpu_result r{};
+        while (ssgpu_batch_poll(bh, &got, &tag, &r) == SSGPU_OK && got) {
+            res[tag] = r;
+            all_conv &= r.converged;
+        }
+    }
+    const std::string err = ssgpu_last_error();
+    ssgpu_batch_destroy(bh);
+    if (st != SSGPU_OK) {
+        set_last_error(err);
+        return st;
+    }
+    return all_conv ? SSGPU_OK : SSGPU_NOT_CONVERGED;
+}
+
+}  // extern "C"

@@ -22,8 +22,7 @@
 #include <thread>
 #include <vector>
 
-#include "sparsesolv_amd.h"
-#include "ssgpu_kernels.h"
+#include "ssgpu_host.h"
 
 namespace ssgpu {
 
@@ -33,46 +32,6 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string& s) { g_last_error = s; }
 
 // ------------------------------------------------------------------ memory
-struct Context;
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    Context* ctx = nullptr;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release();
-    void alloc(Context* c, size_t b);
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct Context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t resident = 0;
-    double* h_scal = nullptr;   // pinned scalar read-back (kScalSlots doubles)
-    DevBuf d_scal;              // device scalar results; the last slot holds the sweep error flag
-    DevBuf d_count;
-    DevBuf d_tickets;           // sync-free sweep tickets (2 ints)
-    std::mutex mu;
-    // matrices created on this context and not yet destroyed; a context whose
-    // destroy arrives while some are alive is freed with the last of them
-    int live_matrices = 0;
-    bool destroy_pending = false;
-    void* owner = nullptr;   // the C-ABI handle holding this context
-    // Krylov work vectors and reduction partials of the solves run on this
-    // stream (grown on demand): solves on different contexts never share them
-    std::vector<std::unique_ptr<DevBuf>> work;
-    DevBuf partial;
-    // block solves: per-column scalars (device + pinned staging), ticket counters
-    DevBuf d_bscal, d_btickets, d_bpartial;
-    void* h_bscal = nullptr;
-};
-
-
-
 void DevBuf::alloc(Context* c, size_t b) {
     release();
     ctx = c;
@@ -95,8 +54,6 @@ void DevBuf::release() {
 // the last slot the flag a sync-free sweep sets when it exceeds its spin
 // limit.  Every scalar read-back copies the whole block, so a solve sees the
 // flag at its next convergence check and stops there.
-constexpr int kScalSlots = 128;
-constexpr int kErrInt = 2 * (kScalSlots - 1);   // the flag, as an int index into the block
 DevBuf& scal_buf(Context* c) {
     if (c->d_scal.bytes == 0) {
         c->d_scal.alloc(c, kScalSlots * sizeof(double));
@@ -118,102 +75,6 @@ const double* read_scalars(Context* c) {
     return c->h_scal;
 }
 
-template <typename T> void upload(Context* c, DevBuf& d, const std::vector<T>& h) {
-    d.alloc(c, sizeof(T) * std::max<size_t>(h.size(), 1));
-    // synchronous: h may be a temporary of the analysis
-    if (!h.empty()) SSGPU_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-}
-
-// Host loops of the analysis whose iterations write disjoint outputs: split
-// [0, n) into contiguous ranges over a few threads (results do not depend on
-// the split).
-template <typename F> void parallel_for(int n, F&& f) {
-    const int hw = (int)std::thread::hardware_concurrency();
-    const int nt = std::max(1, std::min({hw > 0 ? hw : 1, 16, n / 65536}));
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    th.reserve(nt);
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t]() {
-            const int a = (int)((int64_t)n * t / nt), b = (int)((int64_t)n * (t + 1) / nt);
-            for (int i = a; i < b; ++i) f(i);
-        });
-    for (auto& x : th) x.join();
-}
-
-// ------------------------------------------------------------------ structure
-// Rows of a triangular structure stored in level order: level l is the row
-// range levptr[l] .. levptr[l+1] of the permuted CSR (prp, pci), whose row p
-// is original row prow[p]; entries keep the original ascending column order.
-struct LevelCsr {
-    int nlev = 0;
-    int64_t nnz = 0;
-    std::vector<int> levptr;
-    DevBuf prp, pci, prow, map;   // map: source index of each entry
-};
-
-// Sliced-ELL copy of a triangular structure for the sync-free sweeps: rows in
-// level order (position p holds row srow[p]), 64-position slices padded to the
-// slice's longest row, entry k of position (64*s + l) at slice_off[s] + 64*k + l.
-// Off-diagonal entries only, in summation order; the diagonal is kept per
-// position.  Two value sources: the level-ordered factor (f) and the matrix
-// values (a).
-struct SfCsr {
-    int kmax = 0;          // longest row; 0 when a row exceeds 64 entries (level path used)
-    int64_t nell = 0;
-    const int* srow = nullptr;   // level position -> row (the LevelCsr's prow)
-    const LevelCsr* lv = nullptr;   // its levels (host level starts)
-    bool by_level = false;       // one launch per level instead of the sync-free sweep
-    DevBuf slice_off, rlen, ecol, emap_f, dmap_f, emap_a, dmap_a;
-    DevBuf eval, dval, eval32, dval32;
-};
-
-struct Matrix {
-    Context* ctx = nullptr;
-    int dtype = SSGPU_R64;
-    int n = 0;
-    int64_t nnz = 0;
-    size_t vsize = 8;   // bytes per value
-    std::vector<int> h_rp, h_ci;
-    DevBuf rp, ci, val, blk, diag;
-    int nblk = 0;
-    bool lower_diag_ok = true, upper_diag_ok = true;
-    // IC structure: lower(A) in forward-level order; transpose(lower) in backward-level order
-    LevelCsr L, Uic, Us;
-    DevBuf ppos;   // original row -> position in L
-    SfCsr sfL, sfUic, sfUs;   // sync-free sweep copies of lower(A), transpose(lower), upper(A)
-    // IC(0) update lists (built on the first factorization)
-    int lmax_row = 0;          // longest row of lower(A)
-    bool icsym_built = false;
-    int64_t icsym_total = 0;
-    int ic_max_updates = 0;    // largest update list of a row
-    DevBuf ic_po, ic_slot, ic_pos, ic_col;
-    int sweep_grid = 128;     // persistent one-wave workgroups per sweep
-    bool sweeps_tuned = false;   // sweep launch shape measured (first factorization)
-    int spmv_stage_kb = 16;      // SpMV staging (16 or 32 KB), measured at creation
-    int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
-    int block_kc = 0;         // block sweeps: columns per lane (0: 1)
-    int block_grid = 0;       // block sweeps: workgroups per column group (0: about 512 waves in all)
-    bool uic_built = false, us_built = false;
-    // factor / preconditioner values and work space (grown on demand)
-    DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;
-    int factor_pivot_fixes = 0;
-    double factor_accel = 0.0;
-    bool factor_valid = false;
-    // what the sweep values currently hold: 0 nothing, 1 an IC factor
-    // (prep_precision: FP64 only, or FP64 + FP32 copies), 2 SGS values of S A S
-    int prep_kind = 0;
-    int prep_precision = 0;
-    // ABMC-reordered matrix: the library holds P A P^T; solves gather b and x
-    // through perm (new -> old) and return x through iperm (old -> new)
-    bool ordered = false;
-    int ncolors = 0;
-    DevBuf perm, iperm;
-    std::vector<int64_t> vmap;   // stored entry q of P A P^T = caller's entry vmap[q]
-};
 
 // levels of the DAG "row i depends on the entries of its row with index on
 // the solved side"; forward = entries with col < i, backward = col > i.
@@ -1106,6 +967,20 @@ double finish_residual(Solver& S, int solver, const ssgpu_options& o, bool unsca
     return true_res;
 }
 
+// The end of a batched ICCG solve of one system, as solve_factored ends: the
+// true residual and, for an FP32 factor, the refinement passes (context c's
+// stream and work space).
+template <typename T>
+double batch_finish(Context* c, Matrix& m, const ssgpu_options& o, bool conv, double conv_cri, int max_ite,
+                    double normB, const T* b, T* x, int& refinements, int& refine_its) {
+    Solver S(m, c, o);
+    return finish_residual<T>(S, SSGPU_ICCG, o, true, conv, conv_cri, max_ite, normB, b, x, refinements, refine_its);
+}
+template double batch_finish<double>(Context*, Matrix&, const ssgpu_options&, bool, double, int, double,
+                                     const double*, double*, int&, int&);
+template double batch_finish<zc>(Context*, Matrix&, const ssgpu_options&, bool, double, int, double, const zc*, zc*,
+                                 int&, int&);
+
 template <typename T>
 int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite, double normB,
                  const T* b, T* x, double* log, int log_cap, ssgpu_result* res) {
@@ -1642,29 +1517,6 @@ struct PermutedIO {
 // ==================================================================== C-ABI
 using namespace ssgpu;
 
-struct ssgpu_context_s {
-    Context c;
-};
-struct ssgpu_matrix_s {
-    Matrix m;
-};
-
-#define SSGPU_GUARD_BEGIN try {
-#define SSGPU_GUARD_END                                                      \
-    }                                                                        \
-    catch (const Error& e) {                                                 \
-        set_last_error(e.msg);                                               \
-        return e.code;                                                       \
-    }                                                                        \
-    catch (const std::bad_alloc&) {                                          \
-        set_last_error("host allocation failed");                            \
-        return SSGPU_OOM;                                                    \
-    }                                                                        \
-    catch (const std::exception& e) {                                        \
-        set_last_error(e.what());                                            \
-        return SSGPU_HIP_ERROR;                                              \
-    }
-
 extern "C" {
 
 void ssgpu_options_default(ssgpu_options* o) {
@@ -1867,6 +1719,7 @@ int ssgpu_matrix_update_values(ssgpu_matrix mh, const void* values) {
     if (!mh || (!values && mh->m.nnz)) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
     Matrix& m = mh->m;
+    std::lock_guard<std::mutex> lk(m.ctx->mu);
     SSGPU_HIP(hipSetDevice(m.ctx->device));
     std::vector<char> pv;
     if (m.ordered) {   // caller's entry order -> the stored P A P^T order
@@ -1877,7 +1730,10 @@ int ssgpu_matrix_update_values(ssgpu_matrix mh, const void* values) {
     }
     if (m.nnz) SSGPU_HIP(hipMemcpyAsync(m.val.p, values, m.vsize * m.nnz, hipMemcpyHostToDevice, m.ctx->stream));
     SSGPU_HIP(hipStreamSynchronize(m.ctx->stream));
+    // the cached preconditioner (IC factor, or S A S for SGS) belongs to the
+    // old values: cached solves refuse until ssgpu_factorize runs again
     m.factor_valid = false;
+    m.prep_kind = 0;
     return SSGPU_OK;
     SSGPU_GUARD_END
 }

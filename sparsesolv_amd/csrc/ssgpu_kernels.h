@@ -126,6 +126,9 @@ template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d,
 
 // --- sync-free sweeps (ssgpu_sweep.hip) -------------------------------------------
 constexpr int kSfRows = 64;    // rows per chunk = threads per workgroup (one wave)
+// not-ready value of a sync-free sweep's outputs: a signalling-NaN pattern that
+// arithmetic never produces (it only yields quiet NaNs)
+constexpr unsigned long long kSweepSentinel = 0xFFF40000DEADBEEFull;
 // polling passes a sync-free sweep's wave may make without progress before it
 // sets the error flag and leaves (every wave exits, so the grid always drains)
 constexpr long kSweepSpinLimit = 1L << 23;
@@ -165,6 +168,79 @@ void launch_cg_update_cols(hipStream_t s, int kb, int n, int ld, const T* alpha,
                            const T* q, T* x, T* r, double* partial);
 template <typename T>
 void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta, const int* act, const T* z, T* p);
+// --- batched solves (many independent systems, one launch per phase) ----------------
+// Each system keeps its own vectors and scalar state; a launch covers every
+// system of the batch whose status equals the launch's `want` (the others are
+// no-ops).  Every per-system sum runs over the same blocks and the same tree
+// as the one-system kernels above, so a system's iterates are bit-identical to
+// its solo solve.
+enum BatchVec { BV_B = 0, BV_X, BV_P, BV_R, BV_Z, BV_Q, BV_T, BV_BEST, kBatchVecs };
+enum BatchStatus {
+    BS_ACTIVE = 0,      // iterating
+    BS_CONVERGED = 1,   // met the tolerance in the loop
+    BS_STOPPED = 2,     // diverged or reached max_ite
+    BS_EARLY = 3,       // returned from the prologue (b = 0 or the initial residual already small)
+    BS_PENDING = 4,     // added, prologue not run
+    BS_PRIMED = 5,      // prologue passed, first preconditioner apply pending
+    BS_IDLE = 6         // slot holds no system
+};
+struct BatchSys {               // device-resident descriptor of one system
+    int n, nblk, rblk;          // rows; SpMV row blocks; reduction blocks (reduce_blocks(n))
+    const int *blk, *rp, *ci;
+    const void* val;
+    const int *f_srow, *f_soff, *f_rlen, *f_ecol;   // forward sweep: sliced-ELL lower(A) with factor values
+    const void *f_eval, *f_dval;
+    const int *b_srow, *b_soff, *b_rlen, *b_ecol;   // backward sweep: the factor's transpose
+    const void* b_eval;
+    const void* diagD;
+    void* vec[kBatchVecs];
+    double* part;               // this system's partial sums (max(nblk, rblk) x 2 doubles)
+};
+struct BatchState {             // Krylov scalars and bookkeeping of one system (device)
+    double rz[2], alpha[2], beta[2];   // (r, z), alpha, beta; complex as (re, im)
+    double normB, normalizer, abs_conv, best, final_res;
+    double conv_cri, bad_div_val, normalize_const;
+    int max_ite, iterations, bad, status, copy_best;
+    int normalize_type, save_best, diverge_type, bad_div_thres;
+};
+// vector phases of a batched ICCG iteration and prologue
+enum BatchOp {
+    BO_FILL0 = 0,     // vec[a] = 0
+    BO_RESID,         // vec[c] = vec[a] - vec[b]   (b < 0: vec[c] = vec[a])
+    BO_NORM2,         // partial |vec[a]|^2
+    BO_DOT,           // partial vec[a] . vec[b] (non-conjugated)
+    BO_COPY,          // vec[b] = vec[a]
+    BO_SENT,          // vec[a], vec[b] = the sweeps' not-ready sentinel
+    BO_CG_UPDATE,     // x += alpha p; r -= alpha q; partial |r|^2; t, z = sentinel
+    BO_PUPDATE        // best = x when asked; p = p*beta + z
+};
+// scalar phases (one workgroup per system: the fixed-tree finalize of its partials, then)
+enum BatchPhase {
+    BP_PROLOGUE = 0,  // ||r0||: the reference's prologue; PENDING -> PRIMED or EARLY
+    BP_RZ0,           // (r, z); PRIMED -> ACTIVE
+    BP_ALPHA,         // alpha = rz / (p, Ap)  (partials of the SpMV blocks)
+    BP_MONITOR,       // ||r||: convergence, best iterate, divergence, max_ite
+    BP_BETA           // beta = rz' / rz; rz = rz'
+};
+// items: (system, block) pairs; fin: systems, one workgroup each
+template <typename T>
+void launch_batch_spmv(hipStream_t s, int nitems, const int2* items, const BatchSys* sys, const BatchState* st,
+                       int want, int in, int out, int with_dot);
+template <typename T>
+void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, const BatchSys* sys, BatchState* st,
+                      int want, int a, int b, int c);
+// active: count of systems still iterating (decremented as they finish);
+// tickets: the sweep tickets, zeroed by the PROLOGUE and MONITOR phases
+template <typename T>
+void launch_batch_finalize(hipStream_t s, int phase, int nfin, const int* fin, const BatchSys* sys, BatchState* st,
+                           int want, int* active, int* tickets);
+// persistent sync-free sweep over the chunks of every system in `order`
+// (system, chunk) ticket order; mode FWD_DIV_LAST (vec[in] -> vec[out]) or
+// BWD_IC (aux vec[in] -> vec[out]); F the factor's storage type
+template <typename T, typename F>
+void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder, const int2* order, int* ticket,
+                        int* err, long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in,
+                        int out, int max_idle);
 // dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)
 template <typename T, typename F>
 void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);

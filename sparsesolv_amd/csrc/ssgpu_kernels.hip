@@ -82,18 +82,18 @@ __device__ inline int xcd_block(int w, int nb) {
     return xcd * q + min(xcd, rem) + local;
 }
 
+// Row block b of y = A x (and its dot partial): the body of the one-system
+// and the batched SpMV kernels.
 template <typename T, int STAGE_BYTES>
-__global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const int* __restrict__ rp,
-                                              const int* __restrict__ ci, const T* __restrict__ v,
-                                              const T* __restrict__ x, T* __restrict__ y,
-                                              const T* __restrict__ dotv, double* __restrict__ partial) {
+__device__ inline void spmv_block(int b, const int* __restrict__ blk, const int* __restrict__ rp,
+                                  const int* __restrict__ ci, const T* __restrict__ v, const T* __restrict__ x,
+                                  T* __restrict__ y, const T* __restrict__ dotv, double* __restrict__ partial) {
     // products are staged through LDS in pieces of kStage entries: 16 KB
     // keeps 8 workgroups per CU in flight, 32 KB (4 per CU) suits orders whose
     // gathers are scattered (fewer of them compete for L2 at once)
     constexpr int kStage = STAGE_BYTES / (int)sizeof(T) < kSpmvCap ? STAGE_BYTES / (int)sizeof(T) : kSpmvCap;
     static_assert(kStage % kBS == 0 && kSpmvCap % kStage == 0 && kStage >= kSpmvRows, "staging");
     __shared__ T prod[kStage];
-    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int r0 = blk[b], r1 = blk[b + 1];
     const int e0 = rp[r0], e1 = rp[r1];
     const int ne = e1 - e0;
@@ -166,6 +166,14 @@ __global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const
         if (threadIdx.x == 0)
             for (int k = 0; k < K; ++k) partial[b * K + k] = acc[k];
     }
+}
+
+template <typename T, int STAGE_BYTES>
+__global__ __launch_bounds__(kBS) void k_spmv(const int* __restrict__ blk, const int* __restrict__ rp,
+                                              const int* __restrict__ ci, const T* __restrict__ v,
+                                              const T* __restrict__ x, T* __restrict__ y,
+                                              const T* __restrict__ dotv, double* __restrict__ partial) {
+    spmv_block<T, STAGE_BYTES>(xcd_block((int)blockIdx.x, (int)gridDim.x), blk, rp, ci, v, x, y, dotv, partial);
 }
 
 // ---------------------------------------------------------------- sweeps
@@ -283,8 +291,8 @@ __global__ __launch_bounds__(kBS) void k_dot3(int n, const T* __restrict__ a, co
         for (int k = 0; k < 3 * K; ++k) partial[blockIdx.x * 3 * K + k] = acc[k];
 }
 
-__global__ __launch_bounds__(kBS) void k_finalize(int nb, int K, const double* __restrict__ partial,
-                                                  double* __restrict__ out) {
+// out[k] (thread 0) = the fixed-tree sum of partial[b * K + k] over b < nb
+__device__ inline void finalize_parts(int nb, int K, const double* __restrict__ partial, double* __restrict__ out) {
     for (int k = 0; k < K; ++k) {
         double acc[1] = {0.0};
         for (int b = threadIdx.x; b < nb; b += kBS) acc[0] += partial[b * K + k];
@@ -292,6 +300,11 @@ __global__ __launch_bounds__(kBS) void k_finalize(int nb, int K, const double* _
         if (threadIdx.x == 0) out[k] = acc[0];
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kBS) void k_finalize(int nb, int K, const double* __restrict__ partial,
+                                                  double* __restrict__ out) {
+    finalize_parts(nb, K, partial, out);
 }
 
 // ---------------------------------------------------------------- vectors
@@ -447,6 +460,200 @@ template <typename T> __global__ void k_count_nonpos(int n, const T* __restrict_
     if (c) atomicAdd(count, c);
 }
 
+// ---------------------------------------------------------------- batched solves
+// Every kernel below covers many systems: a workgroup reads its (system,
+// block) item, leaves at once when the system's status is not `want`, and
+// otherwise runs exactly what the one-system kernel runs for that block (the
+// same rows, the same grid stride reduce_blocks(n) * kBS, the same tree).
+template <typename T> __device__ inline T* bvec(const BatchSys& S, int k) { return static_cast<T*>(S.vec[k]); }
+template <typename T> __device__ inline T from_parts(const double* d);
+template <> __device__ inline double from_parts<double>(const double* d) { return d[0]; }
+template <> __device__ inline zc from_parts<zc>(const double* d) { return zc{d[0], d[1]}; }
+__device__ inline void to_parts(double v, double* d) { d[0] = v; d[1] = 0.0; }
+__device__ inline void to_parts(zc v, double* d) { d[0] = v.re; d[1] = v.im; }
+template <typename T> __device__ inline T sentinel_value();
+template <> __device__ inline double sentinel_value<double>() { return __longlong_as_double((long long)kSweepSentinel); }
+template <> __device__ inline zc sentinel_value<zc>() {
+    const double s = __longlong_as_double((long long)kSweepSentinel);
+    return zc{s, s};
+}
+
+template <typename T, int STAGE_BYTES>
+__global__ __launch_bounds__(kBS) void k_batch_spmv(const int2* __restrict__ items, const BatchSys* __restrict__ sys,
+                                                    const BatchState* __restrict__ st, int want, int in, int out,
+                                                    int with_dot) {
+    const int2 it = items[xcd_block((int)blockIdx.x, (int)gridDim.x)];
+    if (st[it.x].status != want) return;
+    const BatchSys& S = sys[it.x];
+    const T* x = bvec<T>(S, in);
+    spmv_block<T, STAGE_BYTES>(it.y, S.blk, S.rp, S.ci, static_cast<const T*>(S.val), x, bvec<T>(S, out),
+                               with_dot ? x : nullptr, S.part);
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBS) void k_batch_vec(const int2* __restrict__ items, const BatchSys* __restrict__ sys,
+                                                   const BatchState* __restrict__ st, int want, int a, int b, int c) {
+    const int2 it = items[blockIdx.x];
+    const BatchState& q = st[it.x];
+    if (q.status != want) return;
+    const BatchSys& S = sys[it.x];
+    const int n = S.n, stride = S.rblk * kBS;
+    const int i0 = it.y * kBS + (int)threadIdx.x;
+    constexpr int K = Parts<T>::K;
+    if (OP == BO_FILL0) {
+        T* v = bvec<T>(S, a);
+        for (int i = i0; i < n; i += stride) v[i] = Num<T>::zero();
+    } else if (OP == BO_RESID) {
+        const T* bb = bvec<T>(S, a);
+        const T* qq = b >= 0 ? bvec<T>(S, b) : nullptr;
+        T* r = bvec<T>(S, c);
+        for (int i = i0; i < n; i += stride) r[i] = qq ? Num<T>::sub(bb[i], qq[i]) : bb[i];
+    } else if (OP == BO_COPY) {
+        const T* u = bvec<T>(S, a);
+        T* v = bvec<T>(S, b);
+        for (int i = i0; i < n; i += stride) v[i] = u[i];
+    } else if (OP == BO_SENT) {
+        const T sv = sentinel_value<T>();
+        T* u = bvec<T>(S, a);
+        T* v = bvec<T>(S, b);
+        for (int i = i0; i < n; i += stride) {
+            u[i] = sv;
+            v[i] = sv;
+        }
+    } else if (OP == BO_NORM2) {
+        const T* u = bvec<T>(S, a);
+        double acc[1] = {0.0};
+        for (int i = i0; i < n; i += stride) acc[0] += Num<T>::abs2(u[i]);
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) S.part[it.y] = acc[0];
+    } else if (OP == BO_DOT) {
+        const T* u = bvec<T>(S, a);
+        const T* v = bvec<T>(S, b);
+        double acc[K] = {};
+        for (int i = i0; i < n; i += stride) Parts<T>::put(acc, Num<T>::mul(u[i], v[i]));
+        block_sum<K>(acc);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < K; ++k) S.part[it.y * K + k] = acc[k];
+    } else if (OP == BO_CG_UPDATE) {
+        // as k_cg_update; the sweeps' outputs t and z are reset to the
+        // not-ready sentinel on the way
+        const T alpha = from_parts<T>(q.alpha);
+        const T* p = bvec<T>(S, BV_P);
+        const T* qv = bvec<T>(S, BV_Q);
+        T* x = bvec<T>(S, BV_X);
+        T* r = bvec<T>(S, BV_R);
+        T* t = bvec<T>(S, BV_T);
+        T* z = bvec<T>(S, BV_Z);
+        const T sv = sentinel_value<T>();
+        double acc[1] = {0.0};
+        for (int i = i0; i < n; i += stride) {
+            x[i] = Num<T>::add(x[i], Num<T>::mul(alpha, p[i]));
+            const T ri = Num<T>::sub(r[i], Num<T>::mul(alpha, qv[i]));
+            r[i] = ri;
+            acc[0] += Num<T>::abs2(ri);
+            t[i] = sv;
+            z[i] = sv;
+        }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) S.part[it.y] = acc[0];
+    } else if (OP == BO_PUPDATE) {
+        // as k_cg_pupdate; first the best-iterate copy the monitor asked for
+        const T beta = from_parts<T>(q.beta);
+        const T* z = bvec<T>(S, BV_Z);
+        T* p = bvec<T>(S, BV_P);
+        if (q.copy_best) {
+            const T* x = bvec<T>(S, BV_X);
+            T* best = bvec<T>(S, BV_BEST);
+            for (int i = i0; i < n; i += stride) best[i] = x[i];
+        }
+        for (int i = i0; i < n; i += stride) p[i] = Num<T>::add(Num<T>::mul(p[i], beta), z[i]);
+    }
+}
+
+// One workgroup per system: the fixed-tree finalize of the system's partials
+// (as k_finalize), then the scalar step of the reference's loop for that
+// system, in FP64 / complex FP64 on the device (the same operations the
+// one-system drivers evaluate on the host: MatSolverICCG.cpp:212-237 for the
+// prologue, :259-306 for the loop).
+template <typename T, int PH>
+__global__ __launch_bounds__(kBS) void k_batch_finalize(const int* __restrict__ fin, const BatchSys* __restrict__ sys,
+                                                        BatchState* __restrict__ st, int want, int* __restrict__ active,
+                                                        int* __restrict__ tickets) {
+    if ((PH == BP_PROLOGUE || PH == BP_MONITOR) && blockIdx.x == 0 && threadIdx.x < 2) tickets[threadIdx.x] = 0;
+    const int s = fin[blockIdx.x];
+    if (st[s].status != want) return;
+    const BatchSys& S = sys[s];
+    constexpr int KP = (PH == BP_PROLOGUE || PH == BP_MONITOR) ? 1 : Parts<T>::K;
+    __shared__ double sum[2];
+    finalize_parts(PH == BP_ALPHA ? S.nblk : S.rblk, KP, S.part, sum);
+    if (threadIdx.x != 0) return;
+    BatchState& q = st[s];
+    constexpr double kSmallAbs = 1.0e-20;
+    if (PH == BP_PROLOGUE) {
+        const double pure = sqrt(sum[0]);
+        const double normB = q.normB, conv = q.conv_cri;
+        if (normB <= 1.0e-30 && pure <= kSmallAbs) {
+            q.status = BS_EARLY;
+            atomicSub(active, 1);
+            return;
+        }
+        q.abs_conv = (normB * conv * 0.9 < kSmallAbs ? kSmallAbs : normB * conv * 0.9);
+        const double first = pure / normB;
+        if (first < conv * 0.5 || pure < q.abs_conv * 0.5) {
+            q.status = BS_EARLY;
+            atomicSub(active, 1);
+            return;
+        }
+        if (q.normalize_type == 1) q.normalizer = pure;
+        else if (q.normalize_type == 2) q.normalizer = q.normalize_const;
+        else q.normalizer = normB;
+        q.status = BS_PRIMED;
+    } else if (PH == BP_RZ0) {
+        q.rz[0] = sum[0];
+        q.rz[1] = KP > 1 ? sum[1] : 0.0;
+        if (q.max_ite <= 0) {
+            q.status = BS_STOPPED;
+            atomicSub(active, 1);
+        } else {
+            q.status = BS_ACTIVE;
+        }
+    } else if (PH == BP_ALPHA) {
+        to_parts(Num<T>::div(from_parts<T>(q.rz), from_parts<T>(sum)), q.alpha);
+    } else if (PH == BP_BETA) {
+        const T rz1 = from_parts<T>(sum);
+        to_parts(Num<T>::div(rz1, from_parts<T>(q.rz)), q.beta);
+        to_parts(rz1, q.rz);
+    } else {   // BP_MONITOR
+        const double pn = sqrt(sum[0]);
+        q.iterations += 1;
+        const double normR = pn / q.normalizer;
+        q.final_res = normR;
+        q.copy_best = 0;
+        if (normR < q.conv_cri || pn < q.abs_conv) {
+            q.status = BS_CONVERGED;
+            atomicSub(active, 1);
+            return;
+        }
+        if (normR < q.best) {
+            q.best = normR;
+            if (q.save_best) q.copy_best = 1;
+        }
+        if (q.diverge_type == 1) {
+            if (normR < q.best * q.bad_div_val) q.bad = 0;
+            if (normR >= q.best * q.bad_div_val) q.bad++;
+            if (q.bad >= q.bad_div_thres) {
+                q.status = BS_STOPPED;
+                atomicSub(active, 1);
+                return;
+            }
+        }
+        if (q.iterations >= q.max_ite) {
+            q.status = BS_STOPPED;
+            atomicSub(active, 1);
+        }
+    }
+}
+
 inline int grid_for(int64_t n, int cap = 4096) {
     int64_t g = (n + kBS - 1) / kBS;
     if (g < 1) g = 1;
@@ -471,6 +678,49 @@ void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const i
         hipLaunchKernelGGL((k_spmv<T, 16384>), dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
     else
         hipLaunchKernelGGL((k_spmv<T, 8192>), dim3(nblk), dim3(kBS), 0, s, blk, rp, ci, v, x, y, dotv, partial);
+}
+
+template <typename T>
+void launch_batch_spmv(hipStream_t s, int nitems, const int2* items, const BatchSys* sys, const BatchState* st,
+                       int want, int in, int out, int with_dot) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL((k_batch_spmv<T, 16384>), dim3(nitems), dim3(kBS), 0, s, items, sys, st, want, in, out,
+                       with_dot);
+}
+
+template <typename T>
+void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, const BatchSys* sys, BatchState* st,
+                      int want, int a, int b, int c) {
+    if (nitems <= 0) return;
+#define SSGPU_BV(OP) \
+    hipLaunchKernelGGL((k_batch_vec<T, OP>), dim3(nitems), dim3(kBS), 0, s, items, sys, st, want, a, b, c)
+    switch (op) {
+        case BO_FILL0: SSGPU_BV(BO_FILL0); break;
+        case BO_RESID: SSGPU_BV(BO_RESID); break;
+        case BO_NORM2: SSGPU_BV(BO_NORM2); break;
+        case BO_DOT: SSGPU_BV(BO_DOT); break;
+        case BO_COPY: SSGPU_BV(BO_COPY); break;
+        case BO_SENT: SSGPU_BV(BO_SENT); break;
+        case BO_CG_UPDATE: SSGPU_BV(BO_CG_UPDATE); break;
+        default: SSGPU_BV(BO_PUPDATE); break;
+    }
+#undef SSGPU_BV
+}
+
+template <typename T>
+void launch_batch_finalize(hipStream_t s, int phase, int nfin, const int* fin, const BatchSys* sys, BatchState* st,
+                           int want, int* active, int* tickets) {
+    if (nfin <= 0) return;
+#define SSGPU_BF(PH) \
+    hipLaunchKernelGGL((k_batch_finalize<T, PH>), dim3(nfin), dim3(kBS), 0, s, fin, sys, st, want, active, tickets)
+    switch (phase) {
+        case BP_PROLOGUE: SSGPU_BF(BP_PROLOGUE); break;
+        case BP_RZ0: SSGPU_BF(BP_RZ0); break;
+        case BP_ALPHA: SSGPU_BF(BP_ALPHA); break;
+        case BP_BETA: SSGPU_BF(BP_BETA); break;
+        default: SSGPU_BF(BP_MONITOR); break;
+    }
+#undef SSGPU_BF
 }
 
 template <typename T, typename F>
@@ -600,6 +850,15 @@ template <typename T> void launch_count_nonpos(hipStream_t s, int n, const T* d,
 
 SSGPU_INST(double)
 SSGPU_INST(zc)
+#define SSGPU_BINST(T)                                                                                      \
+    template void launch_batch_spmv<T>(hipStream_t, int, const int2*, const BatchSys*, const BatchState*, int, int, \
+                                       int, int);                                                               \
+    template void launch_batch_vec<T>(hipStream_t, int, int, const int2*, const BatchSys*, BatchState*, int, int, \
+                                      int, int);                                                                \
+    template void launch_batch_finalize<T>(hipStream_t, int, int, const int*, const BatchSys*, BatchState*, int, \
+                                           int*, int*);
+SSGPU_BINST(double)
+SSGPU_BINST(zc)
 template void launch_sweep<double, double>(hipStream_t, int, int, int, const int*, const int*, const double*,
                                            const int*, const double*, const double*, const double*, double*);
 template void launch_sweep<double, float>(hipStream_t, int, int, int, const int*, const int*, const float*,

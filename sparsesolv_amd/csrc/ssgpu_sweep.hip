@@ -32,7 +32,7 @@ namespace ssgpu {
 
 namespace {
 
-constexpr unsigned long long kSentinel = 0xFFF40000DEADBEEFull;
+constexpr unsigned long long kSentinel = kSweepSentinel;
 constexpr int kLvlWaves = 4;   // waves (64-row chunks) per workgroup of the per-level sweep
 
 __device__ inline bool sent(double v) { return (unsigned long long)__double_as_longlong(v) == kSentinel; }
@@ -119,6 +119,69 @@ __device__ inline int take_ticket(int* ticket) {
     return c;
 }
 
+// Chunk c of a sync-free sweep, one lane per row; false when the spin limit
+// stopped the wave (the error flag is set).  spins counts the wave's polling
+// passes since its last progress.
+template <typename T, typename F, int MODE, int KMAX>
+__device__ inline bool sf_chunk(int c, int n, int* __restrict__ err, long spin_limit, const int* __restrict__ srow,
+                                const int* __restrict__ slice_off, const int* __restrict__ rlen,
+                                const int* __restrict__ ecol, const F* __restrict__ eval, const F* __restrict__ dval,
+                                const T* __restrict__ rhs, const T* __restrict__ aux, const T* __restrict__ dd,
+                                int max_idle, T* __restrict__ x, long& spins) {
+    const int lane = threadIdx.x;
+    const int p = c * kSfRows + lane;   // position in level order
+    const bool active = p < n;
+    RowState<T, F, MODE, KMAX> row;
+    row.load(p, active, c, lane, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
+    T g[KMAX];
+    using Mask = typename std::conditional<(KMAX <= 32), unsigned, unsigned long long>::type;
+    constexpr Mask kOne = 1;
+    Mask miss = active ? (row.len >= (int)(8 * sizeof(Mask)) ? ~Mask(0) : ((kOne << row.len) - kOne)) : Mask(0);
+    bool done = !active;
+    int idle = 0;
+    while (true) {
+        __asm__ __volatile__("" ::: "memory");
+        bool progress = false;
+        if (!done && miss) {
+            const Mask want = miss;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                if (want & (kOne << k)) g[k] = gload(x + row.cols[k]);
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                if ((want & (kOne << k)) && !sent(g[k])) miss &= ~(kOne << k);
+            progress = miss != want;
+        }
+        if (!done && miss == 0ull) {
+            gstore(x + row.i, row.result(g));
+            done = true;
+            progress = true;
+        }
+        if (__all(done)) return true;
+        // the limit bounds a stall, not the sweep: a pass with progress
+        // restarts the count (a limit of 0 gives up at the first pass
+        // that does not finish the chunk)
+        const bool any_progress = __any(progress);
+        if (any_progress && spin_limit > 0) spins = 0;
+        if (++spins > spin_limit) {
+            atomicOr(err, 1);
+            return false;
+        }
+        if (any_progress) {
+            idle = 0;
+        } else {
+            // back off while nothing arrives (waves far ahead of the level
+            // front go quiet): 64, 128, then up to 64 * 2^(max_idle-2) clocks
+            if (idle < max_idle) ++idle;
+            if (idle <= 1) __builtin_amdgcn_s_sleep(1);
+            else if (idle == 2) __builtin_amdgcn_s_sleep(2);
+            else if (idle == 3) __builtin_amdgcn_s_sleep(4);
+            else if (idle == 4) __builtin_amdgcn_s_sleep(8);
+            else __builtin_amdgcn_s_sleep(16);
+        }
+    }
+}
+
 template <typename T, typename F, int MODE, int KMAX>
 __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* __restrict__ ticket,
                                                       int* __restrict__ err, long spin_limit,
@@ -128,57 +191,49 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
                                                       const F* __restrict__ dval, const T* __restrict__ rhs,
                                                       const T* __restrict__ aux, const T* __restrict__ dd,
                                                       int max_idle, T* __restrict__ x) {
-    const int lane = threadIdx.x;
     long spins = 0;
     while (true) {
         const int c = take_ticket(ticket);
         if (c >= nchunks) return;
-        const int p = c * kSfRows + lane;   // position in level order
-        const bool active = p < n;
-        RowState<T, F, MODE, KMAX> row;
-        row.load(p, active, c, lane, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
-        T g[KMAX];
-        using Mask = typename std::conditional<(KMAX <= 32), unsigned, unsigned long long>::type;
-        constexpr Mask kOne = 1;
-        Mask miss = active ? (row.len >= (int)(8 * sizeof(Mask)) ? ~Mask(0) : ((kOne << row.len) - kOne)) : Mask(0);
-        bool done = !active;
-        int idle = 0;
-        while (true) {
-            __asm__ __volatile__("" ::: "memory");
-            bool progress = false;
-            if (!done && miss) {
-                const Mask want = miss;
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (want & (kOne << k)) g[k] = gload(x + row.cols[k]);
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if ((want & (kOne << k)) && !sent(g[k])) miss &= ~(kOne << k);
-                progress = miss != want;
-            }
-            if (!done && miss == 0ull) {
-                gstore(x + row.i, row.result(g));
-                done = true;
-                progress = true;
-            }
-            if (__all(done)) break;
-            if (++spins > spin_limit) {
-                atomicOr(err, 1);
-                return;
-            }
-            if (__any(progress)) {
-                idle = 0;
-            } else {
-                // back off while nothing arrives (waves far ahead of the level
-                // front go quiet): 64, 128, then up to 64 * 2^(max_idle-2) clocks
-                if (idle < max_idle) ++idle;
-                if (idle <= 1) __builtin_amdgcn_s_sleep(1);
-                else if (idle == 2) __builtin_amdgcn_s_sleep(2);
-                else if (idle == 3) __builtin_amdgcn_s_sleep(4);
-                else if (idle == 4) __builtin_amdgcn_s_sleep(8);
-                else __builtin_amdgcn_s_sleep(16);
-            }
-        }
+        if (!sf_chunk<T, F, MODE, KMAX>(c, n, err, spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd,
+                                        max_idle, x, spins))
+            return;
+    }
+}
+
+// The sweeps of many systems in one persistent launch (batched solves).
+// order[t] = (system, chunk) is the ticket order: each system's chunks in its
+// own level order, systems interleaved by the level of a chunk's first row.
+// A chunk's inputs lie in chunks of the same system earlier in its level
+// order, hence at smaller tickets, so the launch is deadlock-free like the
+// one-system sweep; the interleave keeps the waves in flight on the level
+// fronts of all systems at once.  Chunks of systems whose status is not
+// `want` are skipped.
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(kSfRows) void k_batch_sweep(int norder, const int2* __restrict__ order,
+                                                         int* __restrict__ ticket, int* __restrict__ err,
+                                                         long spin_limit, const BatchSys* __restrict__ sys,
+                                                         const BatchState* __restrict__ st, int want, int in, int out,
+                                                         int max_idle) {
+    long spins = 0;
+    while (true) {
+        const int t = take_ticket(ticket);
+        if (t >= norder) return;
+        const int2 sc = order[t];
+        if (st[sc.x].status != want) continue;
+        const BatchSys& S = sys[sc.x];
+        bool ok;
+        if (MODE == FWD_DIV_LAST)
+            ok = sf_chunk<T, F, MODE, KMAX>(sc.y, S.n, err, spin_limit, S.f_srow, S.f_soff, S.f_rlen, S.f_ecol,
+                                            static_cast<const F*>(S.f_eval), static_cast<const F*>(S.f_dval),
+                                            static_cast<const T*>(S.vec[in]), nullptr, nullptr, max_idle,
+                                            static_cast<T*>(S.vec[out]), spins);
+        else
+            ok = sf_chunk<T, F, MODE, KMAX>(sc.y, S.n, err, spin_limit, S.b_srow, S.b_soff, S.b_rlen, S.b_ecol,
+                                            static_cast<const F*>(S.b_eval), nullptr, nullptr,
+                                            static_cast<const T*>(S.vec[in]), static_cast<const T*>(S.diagD),
+                                            max_idle, static_cast<T*>(S.vec[out]), spins);
+        if (!ok) return;
     }
 }
 
@@ -300,11 +355,13 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int
                 progress = true;
             }
             if (__all(done)) break;
+            const bool any_progress = __any(progress);   // as in k_sweep_sf: the limit bounds a stall
+            if (any_progress && spin_limit > 0) spins = 0;
             if (++spins > spin_limit) {
                 atomicOr(err, 1);
                 return;
             }
-            if (__any(progress)) {
+            if (any_progress) {
                 idle = 0;
             } else {
                 if (idle < 2) ++idle;
@@ -385,6 +442,23 @@ void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks
     else if (kmax <= 32) SSGPU_SF(32);
     else SSGPU_SF(64);
 #undef SSGPU_SF
+}
+
+template <typename T, typename F, int MODE>
+void batch_sweep_kmax(hipStream_t s, int kmax, int g, int norder, const int2* order, int* ticket, int* err,
+                      long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in, int out,
+                      int max_idle) {
+#define SSGPU_BS(K)                                                                                            \
+    hipLaunchKernelGGL((k_batch_sweep<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, norder, order, ticket, err, \
+                       spin_limit, sys, st, want, in, out, max_idle)
+    if (kmax <= 8) SSGPU_BS(8);
+    else if (kmax <= 12) SSGPU_BS(12);
+    else if (kmax <= 16) SSGPU_BS(16);
+    else if (kmax <= 20) SSGPU_BS(20);
+    else if (kmax <= 24) SSGPU_BS(24);
+    else if (kmax <= 32) SSGPU_BS(32);
+    else SSGPU_BS(64);
+#undef SSGPU_BS
 }
 
 template <typename T, typename F, int MODE, int KC>
@@ -492,6 +566,20 @@ void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int 
                                 rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
+template <typename T, typename F>
+void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder, const int2* order, int* ticket,
+                        int* err, long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in,
+                        int out, int max_idle) {
+    if (norder <= 0) return;
+    const int g = std::max(1, std::min(grid, norder));
+    if (mode == FWD_DIV_LAST)
+        batch_sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in,
+                                             out, max_idle);
+    else
+        batch_sweep_kmax<T, F, BWD_IC>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in, out,
+                                       max_idle);
+}
+
 template <typename T>
 void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
     hipLaunchKernelGGL(k_sf_prepare<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, tickets);
@@ -540,6 +628,13 @@ template void launch_sweep_levels<zc, zc>(hipStream_t, int, int, int, const int*
 template void launch_sweep_levels<zc, fc>(hipStream_t, int, int, int, const int*, const int*, const int*,
                                              const int*, const int*, const fc*, const fc*, const zc*, const zc*,
                                              const zc*, zc*);
+#define SSGPU_BSI(T, F)                                                                                   \
+    template void launch_batch_sweep<T, F>(hipStream_t, int, int, int, int, const int2*, int*, int*, long,       \
+                                           const BatchSys*, const BatchState*, int, int, int, int);
+SSGPU_BSI(double, double)
+SSGPU_BSI(double, float)
+SSGPU_BSI(zc, zc)
+SSGPU_BSI(zc, fc)
 template void launch_gather_pad<double, double>(hipStream_t, int64_t, const int*, const double*, double*);
 template void launch_gather_pad<double, float>(hipStream_t, int64_t, const int*, const double*, float*);
 template void launch_gather_pad<zc, zc>(hipStream_t, int64_t, const int*, const zc*, zc*);

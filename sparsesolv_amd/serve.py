@@ -1,27 +1,35 @@
 """Serving loop for shifted-IC Krylov solves on one GPU.
 
 Requests (a matrix key, its CSR arrays, a right-hand side, the solver and its
-settings) are admitted as they arrive into a fixed number of slots.  Each
-slot is a worker thread with its own HIP stream (a library context), so the
-solves of different slots overlap on the device: a sweep keeps only a few
-hundred waves busy and mostly waits on its dependency chain, which leaves
-the rest of the GPU to the other slots.  Batching is continuous: a slot that
-takes an ICCG request also takes every queued request for the same resident
-system and settings (up to 32) and runs them as one block solve, whose
-sweeps carry all the right-hand sides through the dependency chain at once;
-requests that arrive while it runs wait for the next free slot.  Matrices
-stay resident with their preconditioner (the shifted IC(0) factor, or S A S
-for SGS-MRTR) in a cache keyed by (matrix key, solver family, shift, factor
-precision, ordering); a request for a cached system skips the upload, the
-analysis and the factorization.  A request may ask for the ABMC reordering
-of its system (abmc_block): faster sweeps, a different preconditioner.  The
-cache is bounded by an HBM budget: least recently used systems nobody is
-solving on are evicted to admit a new one, and a request waits while the
-budget is taken by systems in use.
+settings) are admitted as they arrive.  Two kinds of threads share the GPU,
+each with its own HIP stream (a library context):
 
-Each result is exactly what a solo solve against the same factor returns:
-a slot's work never touches another slot's vectors (the library keeps work
-space per context), and the factor is only read.
+* Builders (``slots`` of them) take requests in arrival order.  A request
+  for a system not resident yet uploads the matrix, runs the structural
+  analysis and builds the preconditioner (the shifted IC(0) factor, or
+  S A S for SGS-MRTR) on the builder's stream, so misses build in parallel
+  and overlap with the solves already running.  Matrices stay resident with
+  their preconditioner in a cache keyed by (matrix key, solver family, shift,
+  factor precision, ordering); a request for a cached system skips all that.
+* Batch runners, one per (value type, factor precision), hold a batched ICCG
+  session (ssgpu_batch_*): every ICCG request joins the running batch at the
+  next check (every ``check_every`` iterations), whatever its system; all
+  systems of the batch advance together, one launch per iteration phase for
+  all of them, with the iteration replayed from a hipGraph, and each leaves
+  as it converges.  This is continuous batching: requests that arrive while
+  others are on the GPU enter the batch without waiting for it to drain.
+  IC-MRTR and SGS-MRTR requests (and ICCG with batching off) are solved one
+  at a time on the builder that admitted them.
+
+Each result is exactly what a solo solve against the same factor returns
+(the batched kernels use the one-system kernels' blocks and reduction trees;
+a request's work never touches another's vectors; the factor is only read).
+
+Memory: device bytes are counted on every context of the engine (matrices,
+preconditioners, work vectors, batch tables and vectors), and a system is
+admitted only while that count plus an estimate of the new system fits the
+HBM budget; least recently used systems nobody is solving on are evicted to
+make room, and a request waits while the budget is held by systems in use.
 
 The reference has no serving layer; its solvers refactor on every call
 (MatSolverICCG.cpp:109-118).
@@ -73,6 +81,27 @@ def _system_bytes(nnz: int, n: int, cplx: bool) -> int:
     return int((95 if cplx else 60) * nnz + 64 * n)
 
 
+def _batch_vector_bytes(n: int, cplx: bool) -> int:
+    """Device bytes a system holds while it is in a batch: eight vectors of n
+    values (b, x, p, r, z, Ap, sweep scratch, best iterate) and its partials."""
+    return (16 if cplx else 8) * n * 8 + 64 * (n // 128 + 1024)
+
+
+def rhs_error(rhs, matrix) -> Optional[str]:
+    """Why rhs cannot be solved against matrix (None when it can): it must be
+    one vector of the matrix's size, real or complex, and a complex vector
+    needs a complex matrix (its imaginary part would otherwise be lost)."""
+    a = np.asarray(rhs)
+    if a.ndim != 1 or a.shape[0] != matrix.n:
+        return f"rhs has shape {a.shape}, the matrix needs ({matrix.n},)"
+    if not (np.issubdtype(a.dtype, np.floating) or np.issubdtype(a.dtype, np.complexfloating)
+            or np.issubdtype(a.dtype, np.integer)):
+        return f"rhs dtype {a.dtype} is not numeric"
+    if np.iscomplexobj(a) and not np.issubdtype(matrix.dtype, np.complexfloating):
+        return "complex rhs for a real matrix"
+    return None
+
+
 class _Entry:
     def __init__(self, matrix: capi.Matrix, nbytes: int):
         self.matrix = matrix
@@ -82,25 +111,117 @@ class _Entry:
         self.error: Optional[str] = None
 
 
-MAX_BATCH = 32   # right-hand sides per block solve
+class _Runner:
+    """One batched ICCG session (one value type and factor precision) on its
+    own context and stream; requests join it between checks."""
+
+    def __init__(self, eng: "Engine", complex_: bool, precision: int):
+        self.eng = eng
+        self.complex = complex_
+        self.precision = precision
+        self.cv = threading.Condition()
+        self.inbox: "collections.deque" = collections.deque()
+        self.closing = False
+        self.ctx = None
+        self.occupancy = []          # systems in the batch at each check
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def put(self, item):
+        with self.cv:
+            self.inbox.append(item)
+            self.cv.notify()
+
+    def close(self):
+        with self.cv:
+            self.closing = True
+            self.cv.notify()
+        self.thread.join()
+
+    def resident(self) -> int:
+        c = self.ctx
+        return c.resident_bytes() if c is not None else 0
+
+    def _run(self):
+        eng = self.eng
+        self.ctx = capi.Context(eng.device)
+        batch = capi.Batch(self.ctx, self.complex, self.precision)
+        inflight = {}
+        tag = 0
+        try:
+            while True:
+                with self.cv:
+                    while not self.inbox and not inflight and not self.closing:
+                        self.cv.wait()
+                    if self.closing and not self.inbox and not inflight:
+                        break
+                    items = list(self.inbox)
+                    self.inbox.clear()
+                keep = []
+                for item in items:
+                    r, fut, key, e, hit = item
+                    n = e.matrix.n
+                    # a new system's vectors must fit the budget too, unless the batch is empty
+                    if inflight and not eng._room(_batch_vector_bytes(n, self.complex)):
+                        keep.append(item)
+                        continue
+                    b = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
+                    x = np.zeros(n, e.matrix.dtype)
+                    opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
+                    try:
+                        batch.add_host(e.matrix, b, x, tag, normB=float(np.sqrt(np.sum(np.abs(b) ** 2))),
+                                       conv_cri=r.conv_cri, max_ite=r.max_ite, opts=opts)
+                    except capi.SolverError as ex:
+                        eng._respond(fut, r, None, None, hit, str(ex))
+                        eng._release(key, e)
+                        continue
+                    inflight[tag] = (r, fut, key, e, hit, b, x)
+                    tag += 1
+                if keep:
+                    with self.cv:
+                        self.inbox.extendleft(reversed(keep))
+                if not inflight:
+                    continue
+                err = None
+                try:
+                    running, _ = batch.step(eng.check_every)
+                    self.occupancy.append(running)
+                    eng._note_resident()
+                except capi.SolverError as ex:
+                    err = str(ex)
+                for t, res in batch.poll():
+                    r, fut, key, e, hit, b, x = inflight.pop(t)
+                    if res.status in (capi.OK, capi.NOT_CONVERGED):
+                        eng._respond(fut, r, x, res, hit, None)
+                    else:
+                        eng._respond(fut, r, None, res, hit, err or f"batched solve failed (status {res.status})")
+                    eng._release(key, e)
+        finally:
+            batch.close()
+            self.ctx.close()
+            self.ctx = None
 
 
 class Engine:
-    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: Optional[int] = None, batching: bool = True):
+    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: Optional[int] = None, batching: bool = True,
+                 check_every: int = 4):
         import torch
         self.torch = torch
         self.device = device
         total = torch.cuda.get_device_properties(device).total_memory
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * total)
+        self.batching = batching
+        self.check_every = check_every
         self.lock = threading.Condition()
         self.cache: "collections.OrderedDict[object, _Entry]" = collections.OrderedDict()
-        self.resident = 0
+        self.building = 0            # estimated bytes of systems being built (not yet counted on a context)
         self.peak_resident = 0
-        self.batching = batching
         self.pending: "collections.deque" = collections.deque()
         self.qcv = threading.Condition()
         self.closing = False
-        self.batches = []          # sizes of the solves run (1 = single solve)
+        self.solo_solves = 0
+        self.runners = {}
+        self.contexts = []
         self.workers = []
         for _ in range(slots):
             t = threading.Thread(target=self._worker, daemon=True)
@@ -132,12 +253,19 @@ class Engine:
             self.qcv.notify_all()
         return futs
 
+    @property
+    def batches(self):
+        """Systems in the batch at each check of every runner."""
+        return [k for r in list(self.runners.values()) for k in r.occupancy]
+
     def close(self):
         with self.qcv:
             self.closing = True
             self.qcv.notify_all()
         for t in self.workers:
             t.join()
+        for r in list(self.runners.values()):
+            r.close()
         with self.lock:
             for e in self.cache.values():
                 if e.matrix is not None:
@@ -145,41 +273,49 @@ class Engine:
             self.cache.clear()
 
     # ------------------------------------------------------------ internals
+    def resident(self) -> int:
+        """Device bytes held by the engine's contexts right now."""
+        total = 0
+        for c in list(self.contexts):
+            total += c.resident_bytes()
+        for r in list(self.runners.values()):
+            total += r.resident()
+        return total
+
+    def _note_resident(self):
+        now = self.resident() + self.building
+        if now > self.peak_resident:
+            self.peak_resident = now
+
+    def _room(self, extra: int) -> bool:
+        return self.resident() + self.building + extra <= self.budget
+
+    def _respond(self, fut, r: Request, x, res, hit, err):
+        fut.set_result(Response(x, res, time.perf_counter() - r.arrival, hit, err))
+
     def _cache_key(self, r: Request):
         family = "sgs" if r.solver == capi.MRTR_SGS else "ic"
         return (r.key, family, None if family == "sgs" else float(r.accel), int(r.precision), r.abmc_block)
 
-    def _batch_key(self, r: Request):
-        """Requests with equal keys can share one block solve."""
-        if not self.batching or r.solver != capi.ICCG:
-            return None
-        return (self._cache_key(r), float(r.conv_cri), int(r.max_ite), tuple(sorted(r.opts.items())))
+    def _runner(self, complex_: bool, precision: int) -> _Runner:
+        with self.lock:
+            k = (complex_, int(precision))
+            if k not in self.runners:
+                self.runners[k] = _Runner(self, complex_, precision)
+            return self.runners[k]
 
     def _take(self):
-        """Next request plus the queued ones it can be batched with; None at close."""
         with self.qcv:
             while not self.pending and not self.closing:
                 self.qcv.wait()
             if not self.pending:
                 return None
-            first = self.pending.popleft()
-            batch = [first]
-            bk = self._batch_key(first[0])
-            if bk is not None:
-                keep = collections.deque()
-                while self.pending:
-                    item = self.pending.popleft()
-                    if len(batch) < MAX_BATCH and self._batch_key(item[0]) == bk:
-                        batch.append(item)
-                    else:
-                        keep.append(item)
-                self.pending = keep
-            return batch
+            return self.pending.popleft()
 
     def _acquire(self, r: Request, ctx: capi.Context):
         """The resident system for r (built on the caller's context and stream
-        if needed, so misses in different slots build in parallel), with a user
-        reference."""
+        if needed, so misses in different builders build in parallel), with a
+        user reference."""
         key = self._cache_key(r)
         cplx = np.iscomplexobj(r.values)
         need = _system_bytes(len(r.values), len(r.row_ptr) - 1, cplx)
@@ -194,22 +330,21 @@ class Engine:
                     hit = True
                     break
                 # evict idle systems, least recently used first, until it fits
-                while self.resident + need > self.budget:
+                while not self._room(need):
                     idle = [k for k, v in self.cache.items() if v.users == 0 and v.ready.is_set()]
                     if not idle:
                         break
                     v = self.cache.pop(idle[0])
-                    v.matrix.close()
-                    self.resident -= v.nbytes
-                if self.resident + need <= self.budget:
+                    if v.matrix is not None:
+                        v.matrix.close()
+                if self._room(need) or not any(v.users for v in self.cache.values()):
                     e = _Entry(None, need)
                     e.users = 1
                     self.cache[key] = e
-                    self.resident += need
-                    self.peak_resident = max(self.peak_resident, self.resident)
+                    self.building += need
                     hit = False
                     break
-                self.lock.wait()   # budget held by systems in use
+                self.lock.wait(0.05)   # budget held by systems in use
         if not hit:
             try:
                 m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values, abmc_block=r.abmc_block)
@@ -217,6 +352,9 @@ class Engine:
                 e.matrix = m
             except Exception as ex:   # noqa: BLE001 - reported to every waiter
                 e.error = str(ex)
+            with self.lock:
+                self.building -= need
+                self._note_resident()
             e.ready.set()
         else:
             e.ready.wait()
@@ -230,7 +368,6 @@ class Engine:
             e.users -= 1
             if drop and e.users == 0 and self.cache.get(key) is e:
                 self.cache.pop(key)
-                self.resident -= e.nbytes
                 if e.matrix is not None:
                     e.matrix.close()
             self.lock.notify_all()
@@ -238,44 +375,50 @@ class Engine:
     def _worker(self):
         torch = self.torch
         ctx = capi.Context(self.device)
+        with self.lock:
+            self.contexts.append(ctx)
         stream = torch.cuda.Stream(device=self.device)
         while True:
-            batch = self._take()
-            if batch is None:
+            item = self._take()
+            if item is None:
                 break
-            r0 = batch[0][0]
+            r, fut = item
             try:
-                key, e, hit = self._acquire(r0, ctx)
+                key, e, hit = self._acquire(r, ctx)
             except Exception as ex:  # noqa: BLE001
-                for r, fut in batch:
-                    fut.set_result(Response(None, None, time.perf_counter() - r.arrival, False, str(ex)))
+                self._respond(fut, r, None, None, False, str(ex))
+                continue
+            # the library reads n values of the matrix's type from the rhs: refuse any
+            # other length, and convert the type, before anything reaches the device
+            err = rhs_error(r.rhs, e.matrix)
+            if err:
+                self._respond(fut, r, None, None, hit, err)
+                self._release(key, e)
+                continue
+            if self.batching and r.solver == capi.ICCG and not r.opts.get("is_diag_scale", 0):
+                self._runner(e.matrix.dtype == np.complex128, r.precision).put((r, fut, key, e, hit))
                 continue
             try:
-                opts = capi.default_options(accel_factor=r0.accel, factor_precision=r0.precision, **r0.opts)
+                opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
+                rhs = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
                 with torch.cuda.stream(stream):
-                    d_b = torch.from_numpy(np.ascontiguousarray(np.stack([r.rhs for r, _ in batch]))).to("cuda")
+                    d_b = torch.from_numpy(rhs).to("cuda")
                     d_x = torch.zeros_like(d_b)
                 stream.synchronize()
-                normB = [float(np.sqrt(np.sum(np.abs(r.rhs) ** 2))) for r, _ in batch]
-                if len(batch) == 1:
-                    results = [e.matrix.solve_factored(ctx, r0.solver, d_b.data_ptr(), d_x.data_ptr(),
-                                                       conv_cri=r0.conv_cri, max_ite=r0.max_ite, normB=normB[0],
-                                                       opts=opts)]
-                else:
-                    results = e.matrix.solve_block(ctx, d_b.data_ptr(), d_x.data_ptr(), normB, conv_cri=r0.conv_cri,
-                                                   max_ite=r0.max_ite, opts=opts)
+                res = e.matrix.solve_factored(ctx, r.solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=r.conv_cri,
+                                              max_ite=r.max_ite, normB=float(np.sqrt(np.sum(np.abs(rhs) ** 2))),
+                                              opts=opts)
                 with torch.cuda.stream(stream):
-                    xs = d_x.cpu().numpy()
+                    x = d_x.cpu().numpy()
                 with self.lock:
-                    self.batches.append(len(batch))
-                done = time.perf_counter()
-                for i, (r, fut) in enumerate(batch):
-                    fut.set_result(Response(xs[i].copy(), results[i], done - r.arrival, hit))
+                    self.solo_solves += 1
+                self._respond(fut, r, x, res, hit, None)
             except Exception as ex:  # noqa: BLE001
-                for r, fut in batch:
-                    fut.set_result(Response(None, None, time.perf_counter() - r.arrival, hit, str(ex)))
+                self._respond(fut, r, None, None, hit, str(ex))
             finally:
                 self._release(key, e)
+        with self.lock:
+            self.contexts.remove(ctx)
         ctx.close()
 
 

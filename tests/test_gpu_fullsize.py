@@ -202,3 +202,70 @@ def test_full_size_abmc_meets_tolerance(torch_cuda, ctx):
           flush=True)
     assert conv
     assert true_res <= 1.5e-8
+
+
+# ---------------------------------------------------------------- all three solvers, shift sweep
+def _ref_or_oracle(solver, s, normB, **kw):
+    """The compiled reference on the host (the oracle's port where it is not built)."""
+    if O.reference_available():
+        return O.ref_solve(solver, s.row_ptr, s.col_idx, s.values, s.rhs, normb_mode=0, normB=normB, **kw)
+    return O.solve(solver, s.row_ptr, s.col_idx, s.values, s.rhs, normB=normB, **kw)
+
+
+@pytest.mark.parametrize("case", [
+    ("iccg", capi.ICCG, 1, 1.05),
+    ("iccg", capi.ICCG, 1, 1.3),
+    ("mrtr_ic", capi.MRTR_IC, 0, 1.2),
+    ("mrtr_sgs", capi.MRTR_SGS, 1, 1.2),
+], ids=["iccg_shift1.05", "iccg_shift1.3", "mrtr_ic_shift1.2", "mrtr_sgs"])
+def test_full_size_solvers_against_reference(torch_cuda, big, case):
+    """The n=95 system through every solver (and ICCG at both ends of the
+    1.05-1.3 shift sweep) against the compiled reference: same convergence,
+    iterations within +-2%, first residuals to 1e-10, x within 1e-8.  The
+    solvers whose returned x is the solution of the recurrence must also meet
+    the tolerance in true residual (IC-MRTR in the reference's form returns
+    p = u + t*u's x, MatSolverMRTR_ICver.cpp:302-304, which is not)."""
+    name, code, pt, accel = case
+    s, m = big
+    normB = float(np.linalg.norm(s.rhs))
+    o = capi.default_options(is_save_log=1, accel_factor=accel)
+    conv, x, log, res = m.solve(code, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
+    c_r, x_r, log_r = _ref_or_oracle("iccg" if name == "iccg" else "mrtr", s, normB, conv_cri=1e-8,
+                                     max_ite=20000, accel_factor=accel, precondition_type=pt)
+    it_g, it_r = len(log) - 1, len(log_r) - 1
+    rel = float(np.linalg.norm(x - x_r) / np.linalg.norm(x_r))
+    true_res = float(np.linalg.norm(s.rhs - s.scipy() @ x) / normB)
+    print(f"full size {name} shift {accel}: iterations gpu {it_g} reference {it_r}, x relative difference "
+          f"{rel:.3e}, true residual {true_res:.3e}, gpu {res.setup_ms + res.iterate_ms:.0f} ms", flush=True)
+    assert conv == c_r
+    assert abs(it_g - it_r) <= max(1, int(0.02 * it_r)), (it_g, it_r)
+    np.testing.assert_allclose(log[:6], log_r[:6], rtol=1e-10)
+    assert rel <= 1e-8, rel
+    if name != "mrtr_ic":
+        assert true_res <= 1.5e-8
+
+
+# ---------------------------------------------------------------- tree order at full size
+@pytest.mark.parametrize("which", ["iccg_real_n95", "cocg_complex_n64"])
+def test_full_size_bit_identical_to_tree_order_oracle(torch_cuda, big, big_c, which):
+    """With the oracle's reductions switched to the GPU's fixed tree, the
+    full-size solves are bit-identical: the same x, the same residual log and
+    the same iteration count (both sides get the same normB).  So at full size
+    too the GPU's results differ from the reference's only through the order
+    of its dot products and norms.  The oracle needs about a minute here."""
+    if which == "iccg_real_n95":
+        s, m = big
+        accel = 1.2
+    else:
+        s, m = big_c
+        accel = 1.1
+    normB = float(np.linalg.norm(s.rhs))
+    o = capi.default_options(is_save_log=1, accel_factor=accel)
+    conv, x, log, res = m.solve(capi.ICCG, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
+    print(f"tree order {which}: gpu iterations {res.iterations}; running the oracle", flush=True)
+    with O.reduction_order(1):
+        c_o, x_o, log_o = O.solve("iccg", s.row_ptr, s.col_idx, s.values, s.rhs, conv_cri=1e-8, max_ite=20000,
+                                  normB=normB, accel_factor=accel, precondition_type=1)
+    assert conv == c_o
+    assert np.array_equal(log, log_o)
+    assert np.array_equal(x, x_o)

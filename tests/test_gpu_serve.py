@@ -195,28 +195,102 @@ def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
     m.close()
 
 
-def test_serving_batches_same_system_requests(torch_cuda, ctx):
-    """Requests queued together for one system run as one block solve; each
-    response still equals its solo solve."""
-    s = voxel.curlcurl(10, seed=8)
-    rhs = [_rhs(s, 200 + k) for k in range(9)]
+def test_serving_batches_requests_together(torch_cuda, ctx):
+    """ICCG requests queued together, for one system and for different
+    systems, run in one batched session: the batch holds several of them at
+    once, and each response still equals its solo solve."""
+    systems = [voxel.curlcurl(10, seed=8), voxel.curlcurl(7, seed=9), voxel.curlcurl(12, seed=10)]
+    plan = [(k % 3, _rhs(systems[k % 3], 200 + k)) for k in range(9)]
     eng = serve.Engine(slots=2)
-    first = eng.submit(serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=rhs[0],
-                                     accel=1.1)).result(timeout=120)
-    assert first.error is None
-    eng.batches.clear()
-    futs = eng.submit_many([serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
-                                          accel=1.1) for b in rhs])
+    futs = eng.submit_many([serve.Request(key=i, row_ptr=systems[i].row_ptr, col_idx=systems[i].col_idx,
+                                          values=systems[i].values, rhs=b, accel=1.1) for i, b in plan])
     resp = [f.result(timeout=120) for f in futs]
-    batches = list(eng.batches)
+    occupancy = list(eng.batches)
     eng.close()
-    print(f"batches {batches}", flush=True)
-    assert sum(batches) == len(rhs) and max(batches) > 1
-    for b, r in zip(rhs, resp):
+    print(f"batch occupancy at the checks: {occupancy}", flush=True)
+    assert max(occupancy) > 1
+    for (i, b), r in zip(plan, resp):
         assert r.error is None, r.error
-        x_solo, r_solo = _solo(torch_cuda, ctx, s, b, capi.ICCG, 1.1)
+        x_solo, r_solo = _solo(torch_cuda, ctx, systems[i], b, capi.ICCG, 1.1)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo)
+
+
+def test_serving_rejects_bad_rhs(torch_cuda, ctx):
+    """A right-hand side of the wrong length, or complex for a real system, is
+    answered with an error; the other requests are served."""
+    s = voxel.curlcurl(6, seed=3)
+    eng = serve.Engine(slots=2)
+    bad = [np.zeros(s.n + 1), np.zeros(s.n, np.complex128), np.zeros((s.n, 2))]
+    futs = [eng.submit(serve.Request(key="s", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                     accel=1.1)) for b in bad + [s.rhs]]
+    resp = [f.result(timeout=120) for f in futs]
+    eng.close()
+    assert all(r.error for r in resp[:3])
+    assert resp[3].error is None and resp[3].result.converged
+
+
+def test_serving_poisson_stream_against_oracle(torch_cuda, ctx):
+    """A config-5-shaped stream scaled to test time: Poisson arrivals of 40
+    requests, grid edge log-uniform in 4..16, 70% real / 30% complex, 40%
+    reusing an earlier system with a new right-hand side, FP64 and FP32
+    factors, and an HBM budget of a few systems so the cache evicts.  FP64
+    responses equal the oracle in the GPU's reduction order bit for bit; FP32
+    responses equal their solo solve and meet the tolerance in true residual."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    seen, plan, cache = [], [], {}
+    for k in range(40):
+        if seen and rng.uniform() < 0.4:
+            key = seen[int(rng.integers(len(seen)))]
+            rhs_seed = 1000 + k
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(4), np.log(16)))))
+            key = (k, n, bool(rng.uniform() < 0.3))
+            seen.append(key)
+            rhs_seed = None
+        if key not in cache:
+            cache[key] = voxel.eddy_current(key[1], seed=key[0]) if key[2] else voxel.curlcurl(key[1], seed=key[0])
+        s = cache[key]
+        b = s.rhs if rhs_seed is None else _rhs(s, rhs_seed)
+        prec = capi.FACTOR_F32 if k % 3 == 2 else capi.FACTOR_F64
+        plan.append((key, b, prec))
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, size=len(plan)))
+    biggest = max(serve._system_bytes(s.nnz, s.n, s.is_complex) for s in cache.values())
+    eng = serve.Engine(slots=3, hbm_budget=int(4 * biggest) + (64 << 20))
+    import time
+    t0 = time.perf_counter()
+    futs = []
+    for (key, b, prec), at in zip(plan, arrivals):
+        d = t0 + at - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        s = cache[key]
+        futs.append(eng.submit(serve.Request(key=key, row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b,
+                                             accel=1.1, precision=prec, max_ite=5000)))
+    resp = [f.result(timeout=300) for f in futs]
+    occupancy = list(eng.batches)
+    eng.close()
+    lat = serve.percentiles([r.latency_s for r in resp])
+    print(f"poisson stream: {len(resp)} requests, {len(cache)} systems, {sum(r.cache_hit for r in resp)} hits, "
+          f"max batch {max(occupancy)}, p50 {1e3 * lat['p50']:.0f} ms p99 {1e3 * lat['p99']:.0f} ms", flush=True)
+    assert max(occupancy) > 1
+    for (key, b, prec), r in zip(plan, resp):
+        assert r.error is None, r.error
+        s = cache[key]
+        assert r.result.converged
+        nb = float(np.sqrt(np.sum(np.abs(b) ** 2)))
+        assert np.linalg.norm(b - s.scipy() @ r.x) / nb <= 2e-8
+        if prec == capi.FACTOR_F64:
+            with O.reduction_order(1):
+                c_o, x_o, log_o = O.solve("iccg", s.row_ptr, s.col_idx, s.values, b, conv_cri=1e-8, max_ite=5000,
+                                          normB=nb, accel_factor=1.1)
+            assert r.result.iterations == len(log_o) - 1
+            assert np.array_equal(r.x, x_o), key
+        else:
+            x_solo, r_solo = _solo(torch_cuda, ctx, s, b, capi.ICCG, 1.1, prec)
+            assert r.result.iterations == r_solo.iterations
+            assert np.array_equal(r.x, x_solo), key
 
 
 def test_serving_abmc_requests(torch_cuda, ctx):
