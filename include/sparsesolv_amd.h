@@ -178,7 +178,7 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
                       int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X,
                       ssgpu_result* res);
 
-/* Batched ICCG / COCG over many independent systems (the caller's own
+/* Batched ICCG / COCG, or IC-MRTR, over many independent systems (the caller's own
  * matrices, of any sizes; one matrix may appear several times with different
  * right-hand sides).  No counterpart in the reference, which solves one system
  * per call with its Krylov loop on the host (MatSolverICCG.cpp:257-318): here
@@ -188,8 +188,10 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
  * x, iteration count and residuals as ssgpu_solve_factored on it alone (same
  * factor, same options).
  *
- * A batch is a session on one context: systems join with ssgpu_batch_add
- * (the matrix factorized by ssgpu_factorize, ICCG only, no diagonal scaling;
+ * A batch is a session on one context for one solver (SSGPU_ICCG or
+ * SSGPU_MRTR_IC; MatSolverMRTR_ICver.cpp:183-373 with its quirks, or the
+ * standard direction with mrtr_direction_fix): systems join with ssgpu_batch_add
+ * (the matrix factorized by ssgpu_factorize, no diagonal scaling;
  * d_b and d_x device vectors of the matrix's n entries that stay valid until
  * the system's result is polled; d_x is the start vector when
  * opt->is_init == 0 and receives the solution) and enter the loop at the
@@ -204,7 +206,7 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
  * from a step) every system of the batch has been dropped with that status;
  * the batch and the matrices stay usable. */
 typedef struct ssgpu_batch_s* ssgpu_batch;
-int ssgpu_batch_create(ssgpu_context ctx, int dtype, int factor_precision, ssgpu_batch* out);
+int ssgpu_batch_create(ssgpu_context ctx, int solver, int dtype, int factor_precision, ssgpu_batch* out);
 int ssgpu_batch_add(ssgpu_batch b, ssgpu_matrix m, const ssgpu_options* opt, double conv_cri, int max_ite,
                     double normB, const void* d_b, void* d_x, int64_t tag);
 /* Same with b and x in host memory (the reference's pointer overloads,
@@ -215,11 +217,14 @@ int ssgpu_batch_add_host(ssgpu_batch b, ssgpu_matrix m, const ssgpu_options* opt
                          double normB, const void* b_host, void* x_host, int64_t tag);
 int ssgpu_batch_step(ssgpu_batch b, int iterations, int* running, int* finished);
 int ssgpu_batch_poll(ssgpu_batch b, int* got, int64_t* tag, ssgpu_result* res);
+/* Systems in the batch, iterations run so far, and the rebuilds whose graph
+ * capture the runtime refused (those steps ran the same kernels eagerly). */
+int ssgpu_batch_info(ssgpu_batch b, int* systems, int64_t* iterations_run, int* graph_fallbacks);
 int ssgpu_batch_destroy(ssgpu_batch b);
 /* One-shot form: nsys systems with shared options, tolerance and iteration
  * limit; res[i] for system i.  Returns SSGPU_OK when every system converged,
  * SSGPU_NOT_CONVERGED otherwise. */
-int ssgpu_solve_batch(ssgpu_context ctx, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
+int ssgpu_solve_batch(ssgpu_context ctx, int solver, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
                       double conv_cri, int max_ite, const double* normB, const void* const* d_b, void* const* d_x,
                       ssgpu_result* res);
 

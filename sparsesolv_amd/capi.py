@@ -80,13 +80,14 @@ def lib():
         L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
         L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_solve_block.argtypes = [V, V, P(Options), D, I, I, V, V, V, P(Result)]
-        L.ssgpu_batch_create.argtypes = [V, I, I, P(V)]
+        L.ssgpu_batch_create.argtypes = [V, I, I, I, P(V)]
         L.ssgpu_batch_add.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
         L.ssgpu_batch_add_host.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
         L.ssgpu_batch_step.argtypes = [V, I, P(I), P(I)]
         L.ssgpu_batch_poll.argtypes = [V, P(I), P(ctypes.c_int64), P(Result)]
+        L.ssgpu_batch_info.argtypes = [V, P(I), P(ctypes.c_int64), P(I)]
         L.ssgpu_batch_destroy.argtypes = [V]
-        L.ssgpu_solve_batch.argtypes = [V, I, P(V), P(Options), D, I, V, P(V), P(V), P(Result)]
+        L.ssgpu_solve_batch.argtypes = [V, I, I, P(V), P(Options), D, I, V, P(V), P(V), P(Result)]
         L.ssgpu_spmv_device.argtypes = [V, V, V]
         L.ssgpu_ic_factor_host.argtypes = [V, D, V, V, V, V, P(I)]
         L.ssgpu_ic_apply_device.argtypes = [V, V, V]
@@ -288,13 +289,13 @@ class Matrix:
 
 
 class Batch:
-    """A batched ICCG / COCG session on one context (ssgpu_batch_*): systems
-    join with add(), advance with step(), and leave through poll()."""
+    """A batched ICCG / COCG or IC-MRTR session on one context (ssgpu_batch_*):
+    systems join with add(), advance with step(), and leave through poll()."""
 
-    def __init__(self, ctx: Context, complex_: bool = False, precision=FACTOR_F64):
+    def __init__(self, ctx: Context, complex_: bool = False, precision=FACTOR_F64, solver=ICCG):
         self.ctx = ctx
         self.h = ctypes.c_void_p()
-        check(lib().ssgpu_batch_create(ctx.h, C128 if complex_ else R64, precision, ctypes.byref(self.h)))
+        check(lib().ssgpu_batch_create(ctx.h, solver, C128 if complex_ else R64, precision, ctypes.byref(self.h)))
 
     def add(self, m: "Matrix", d_b: int, d_x: int, tag: int, *, normB, conv_cri=1e-8, max_ite=10000, opts=None):
         o = opts or default_options()
@@ -326,6 +327,12 @@ class Batch:
                 return out
             out.append((tag.value, res))
 
+    def info(self):
+        """(systems in the batch, iterations run, graph captures that fell back to eager launches)"""
+        n, its, fb = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
+        check(lib().ssgpu_batch_info(self.h, ctypes.byref(n), ctypes.byref(its), ctypes.byref(fb)))
+        return n.value, its.value, fb.value
+
     def close(self):
         if self.h:
             lib().ssgpu_batch_destroy(self.h)
@@ -338,8 +345,8 @@ class Batch:
             pass
 
 
-def solve_batch(ctx: Context, mats, d_bs, d_xs, normBs, *, conv_cri=1e-8, max_ite=10000, opts=None):
-    """One-shot batched ICCG / COCG (ssgpu_solve_batch); one Result per system."""
+def solve_batch(ctx: Context, mats, d_bs, d_xs, normBs, *, conv_cri=1e-8, max_ite=10000, opts=None, solver=ICCG):
+    """One-shot batched ICCG / COCG or IC-MRTR (ssgpu_solve_batch); one Result per system."""
     k = len(mats)
     o = opts or default_options()
     M = (ctypes.c_void_p * k)(*[m.h.value for m in mats])
@@ -347,7 +354,7 @@ def solve_batch(ctx: Context, mats, d_bs, d_xs, normBs, *, conv_cri=1e-8, max_it
     Xp = (ctypes.c_void_p * k)(*d_xs)
     nb = np.ascontiguousarray(normBs, dtype=np.float64)
     res = (Result * k)()
-    st = lib().ssgpu_solve_batch(ctx.h, k, M, ctypes.byref(o), conv_cri, max_ite, nb.ctypes.data, Bp, Xp, res)
+    st = lib().ssgpu_solve_batch(ctx.h, solver, k, M, ctypes.byref(o), conv_cri, max_ite, nb.ctypes.data, Bp, Xp, res)
     check(st, allow=(OK, NOT_CONVERGED))
     return list(res)
 

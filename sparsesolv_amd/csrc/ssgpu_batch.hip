@@ -1,14 +1,14 @@
-// Batched ICCG / COCG: many independent systems (uneven sizes, any mix of
-// matrices, several right-hand sides of one matrix) advanced together, one
-// launch per phase of the iteration for all of them.
+// Batched ICCG / COCG and IC-MRTR: many independent systems (uneven sizes,
+// any mix of matrices, several right-hand sides of one matrix) advanced
+// together, one launch per phase of the iteration for all of them.
 //
 // The reference solves one system per call and keeps the Krylov loop on the
 // host (MatSolverICCG.cpp:257-318).  Here the loop's scalars live on the
 // device, one BatchState per system: the finalize kernels evaluate alpha,
 // beta, the residual norm and the reference's convergence bookkeeping for
 // every system at once, and a system that converges, diverges or reaches its
-// iteration limit turns every later launch into a no-op for itself.  The nine
-// launches of one iteration are captured once into a hipGraph and replayed;
+// iteration limit turns every later launch into a no-op for itself.  The
+// launches of one iteration are built once into a hipGraph and replayed;
 // the host looks at the batch only between replays (one small read-back),
 // where finished systems leave and newly added ones join (continuous
 // batching).  The sweeps of all systems run in one persistent launch
@@ -52,6 +52,7 @@ struct Slot {
 
 struct Batch {
     Context* c = nullptr;
+    int solver = SSGPU_ICCG;   // SSGPU_ICCG or SSGPU_MRTR_IC
     int dtype = SSGPU_R64;
     int precision = SSGPU_FACTOR_F64;
     long spin_limit = kSweepSpinLimit;
@@ -70,6 +71,7 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     std::deque<std::pair<int64_t, ssgpu_result>> done;
     int64_t iterations_run = 0;
+    int capture_failures = 0;   // rebuilds whose graph capture fell back to eager launches
 
     int* d_active() { return d_flags.as<int>(); }
     int* d_err() { return d_flags.as<int>() + 1; }
@@ -139,65 +141,107 @@ template <typename T> struct Runner {
     const int2* red() { return B.d_items_red.as<int2>(); }
     const int* fin() { return B.d_fin.as<int>(); }
 
-    void vec(int op, int want, int a, int b = -1, int c = -1) {
-        launch_batch_vec<T>(s, op, B.n_red, red(), sys(), st(), want, a, b, c);
+    // launches go to the stream, or into the graph being built when rec is set
+    GraphRec* rec = nullptr;
+    void vec(int op, int want, int a, int b = -1, int c = -1, bool reset_tickets = false) {
+        launch_batch_vec<T>(s, op, B.n_red, red(), sys(), st(), want, a, b, c, reset_tickets ? B.d_tickets() : nullptr,
+                            rec);
     }
     void finalize(int phase, int want) {
-        launch_batch_finalize<T>(s, phase, B.n_fin, fin(), sys(), st(), want, B.d_active(), B.d_tickets());
+        launch_batch_finalize<T>(s, phase, B.n_fin, fin(), sys(), st(), want, B.d_active(), B.d_tickets(), rec);
     }
     void sweeps(int want, int in, int mid, int out) {
         int* tk = B.d_tickets();
         if (B.precision == SSGPU_FACTOR_F32) {
             launch_batch_sweep<T, F>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
-                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2);
+                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
             launch_batch_sweep<T, F>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
-                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out,
-                                     2);
+                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
         } else {
             launch_batch_sweep<T, T>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
-                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2);
+                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
             launch_batch_sweep<T, T>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
-                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out,
-                                     2);
+                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
         }
+    }
+    void spmv(int want, int in, int out, int with_dot) {
+        launch_batch_spmv<T>(s, B.n_mv, mv(), sys(), st(), want, in, out, with_dot, rec);
     }
 
     // the prologue of the systems added since the last step: ||r0|| and the
-    // reference's early exits, z0 = M^-1 r0, p = z0, (r, z)
+    // reference's early exits, then
+    //   ICCG: z0 = M^-1 r0, p = z0, (r, z)   (MatSolverICCG.cpp:212-251)
+    //   IC-MRTR: u = M^-1 r0, y = -r0, z = M^-1 y   (MatSolverMRTR_ICver.cpp:224-262)
     void prologue() {
         vec(BO_NORM2, BS_PENDING, BV_R);
         finalize(BP_PROLOGUE, BS_PENDING);
-        vec(BO_SENT, BS_PRIMED, BV_T, BV_P);
-        sweeps(BS_PRIMED, BV_R, BV_T, BV_P);
-        vec(BO_COPY, BS_PRIMED, BV_P, BV_Z);
-        vec(BO_DOT, BS_PRIMED, BV_R, BV_Z);
-        finalize(BP_RZ0, BS_PRIMED);
+        if (B.solver == SSGPU_ICCG) {
+            vec(BO_SENT, BS_PRIMED, BV_T, BV_P);
+            sweeps(BS_PRIMED, BV_R, BV_T, BV_P);
+            vec(BO_COPY, BS_PRIMED, BV_P, BV_Z);
+            vec(BO_DOT, BS_PRIMED, BV_R, BV_Z);
+            finalize(BP_RZ0, BS_PRIMED);
+        } else {
+            vec(BO_SENT, BS_PRIMED, BV_T, BV_U);
+            sweeps(BS_PRIMED, BV_R, BV_T, BV_U);
+            vec(BO_NEG, BS_PRIMED, BV_R, BV_Y);
+            vec(BO_SENT, BS_PRIMED, BV_T, BV_Z, -1, true);
+            sweeps(BS_PRIMED, BV_Y, BV_T, BV_Z);
+            vec(BO_SENT, BS_PRIMED, BV_T, BV_W);
+            finalize(BP_ACTIVATE, BS_PRIMED);
+        }
     }
-    // one iteration of MatSolverICCG.cpp:257-318 for every active system
+    // one iteration for every active system: MatSolverICCG.cpp:257-318, or
+    // MatSolverMRTR_ICver.cpp:270-360
     void iteration() {
-        launch_batch_spmv<T>(s, B.n_mv, mv(), sys(), st(), BS_ACTIVE, BV_P, BV_Q, 1);
-        finalize(BP_ALPHA, BS_ACTIVE);
-        vec(BO_CG_UPDATE, BS_ACTIVE, 0);
-        finalize(BP_MONITOR, BS_ACTIVE);
-        sweeps(BS_ACTIVE, BV_R, BV_T, BV_Z);
-        vec(BO_DOT, BS_ACTIVE, BV_R, BV_Z);
-        finalize(BP_BETA, BS_ACTIVE);
-        vec(BO_PUPDATE, BS_ACTIVE, 0);
+        if (B.solver == SSGPU_ICCG) {
+            spmv(BS_ACTIVE, BV_P, BV_Q, 1);
+            finalize(BP_ALPHA, BS_ACTIVE);
+            vec(BO_CG_UPDATE, BS_ACTIVE, 0);
+            finalize(BP_MONITOR, BS_ACTIVE);
+            sweeps(BS_ACTIVE, BV_R, BV_T, BV_Z);
+            vec(BO_DOT, BS_ACTIVE, BV_R, BV_Z);
+            finalize(BP_BETA, BS_ACTIVE);
+            vec(BO_PUPDATE, BS_ACTIVE, 0);
+        } else {
+            spmv(BS_ACTIVE, BV_U, BV_Q, 0);
+            sweeps(BS_ACTIVE, BV_Q, BV_T, BV_W);
+            vec(BO_DOT3, BS_ACTIVE, 0);
+            finalize(BP_MRTR_COEF, BS_ACTIVE);
+            vec(BO_MRTR_UPDATE, BS_ACTIVE, 0);
+            finalize(BP_MONITOR, BS_ACTIVE);
+            vec(BO_MRTR_UPDATE2, BS_ACTIVE, 0);
+        }
     }
 
+    // Build the graph of one iteration node by node (hipGraphAddKernelNode in
+    // launch order, each node depending on the one before).  No stream is put
+    // in capture mode, so calls other threads make meanwhile (allocations,
+    // frees, synchronizations) cannot meet a capturing stream.  A graph the
+    // runtime refuses leaves the batch on eager launches of the same kernels
+    // (the same bits), counted in capture_failures.
     void capture() {
         B.drop_graph();
         if (!B.use_graph || B.n_fin == 0) return;
-        SSGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        iteration();
         hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &g);
+        hipError_t e = hipGraphCreate(&g, 0);
+        if (e == hipSuccess) {
+            GraphRec r;
+            r.graph = g;
+            rec = &r;
+            iteration();
+            rec = nullptr;
+            e = r.err;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             if (g) (void)hipGraphDestroy(g);
-            throw Error{HIP_ERROR, std::string("graph capture of the batched iteration failed: ") + hipGetErrorString(e)};
+            B.exec = nullptr;
+            ++B.capture_failures;
+            return;
         }
         B.graph = g;
-        SSGPU_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
     }
 
     // rebuild the device tables for the systems in the batch, run the
@@ -285,13 +329,17 @@ template <typename T> struct Runner {
         T* x = static_cast<T*>(d.vec[BV_X]);
         const T* b = static_cast<const T*>(d.vec[BV_B]);
         const bool conv = q.status == BS_CONVERGED || q.status == BS_EARLY;
+        // IC-MRTR keeps its iterate apart and returns it at the end, except from
+        // the prologue's early exit (MatSolverMRTR_ICver.cpp:230-240, 362-367)
+        if (B.solver == SSGPU_MRTR_IC && q.status != BS_EARLY)
+            launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_XI]), x);
         // MatSolverICCG.cpp:315-317: on failure with save-best, the best iterate
         // (the current x when the last check had just saved it)
         if (!conv && sl.opt.is_save_best && !q.copy_best)
             launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_BEST]), x);
         int refinements = 0, refine_its = 0;
-        const double true_res = batch_finish<T>(B.c, m, sl.opt, conv, sl.conv_cri, sl.max_ite, sl.normB, b, x,
-                                                refinements, refine_its);
+        const double true_res = batch_finish<T>(B.c, m, B.solver, sl.opt, conv, sl.conv_cri, sl.max_ite, sl.normB, b,
+                                                x, refinements, refine_its);
         if (sl.own_bx) {   // x back in the caller's order, to the caller's device or host array
             T* dst = sl.host_io ? static_cast<T*>(d.vec[BV_T]) : static_cast<T*>(sl.user_x);
             if (m.ordered) launch_gather<T, T>(s, m.n, m.iperm.as<int>(), x, dst);
@@ -371,7 +419,8 @@ template <typename T> struct Runner {
         const size_t vbytes = sizeof(T) * (size_t)n;
         if (!sl.vecs) sl.vecs.reset(new DevBuf());
         if (!sl.part) sl.part.reset(new DevBuf());
-        if (sl.vecs->bytes < vbytes * kBatchVecs) sl.vecs->alloc(c, vbytes * kBatchVecs);
+        const int nvec = B.solver == SSGPU_ICCG ? kBatchVecsIccg : kBatchVecs;
+        if (sl.vecs->bytes < vbytes * nvec) sl.vecs->alloc(c, vbytes * nvec);
         const int rblk = reduce_blocks(n);
         const size_t pbytes = sizeof(double) * 2 * (size_t)std::max(m.nblk, rblk);
         if (sl.part->bytes < pbytes) sl.part->alloc(c, pbytes);
@@ -397,7 +446,7 @@ template <typename T> struct Runner {
         d.b_eval = f32 ? m.sfUic.eval32.p : m.sfUic.eval.p;
         d.diagD = m.diagD.p;
         char* base = static_cast<char*>(sl.vecs->p);
-        for (int k = 0; k < kBatchVecs; ++k) d.vec[k] = base + vbytes * k;
+        for (int k = 0; k < kBatchVecs; ++k) d.vec[k] = k < nvec ? base + vbytes * k : nullptr;
         d.part = sl.part->as<double>();
         sl.own_bx = m.ordered || host_io;
         sl.host_io = host_io;
@@ -422,19 +471,35 @@ template <typename T> struct Runner {
             d.vec[BV_B] = const_cast<void*>(ub);
             d.vec[BV_X] = ux;
         }
-        // the first lines of iccg_main, on this system alone
+        // the first lines of iccg_main / mrtr_main, on this system alone
         T* x = static_cast<T*>(d.vec[BV_X]);
         const T* b = static_cast<const T*>(d.vec[BV_B]);
         T* r = static_cast<T*>(d.vec[BV_R]);
         T* q = static_cast<T*>(d.vec[BV_Q]);
-        if (o.is_init) launch_fill<T>(s, n, x, T{});
         if (o.is_save_best) launch_fill<T>(s, n, static_cast<T*>(d.vec[BV_BEST]), T{});
-        if (!o.is_init) {
-            launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), x, q, nullptr,
-                           nullptr, m.spmv_stage_kb);
-            launch_residual<T>(s, n, b, q, r);
+        if (B.solver == SSGPU_ICCG) {
+            if (o.is_init) launch_fill<T>(s, n, x, T{});
+            if (!o.is_init) {
+                launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), x, q,
+                               nullptr, nullptr, m.spmv_stage_kb);
+                launch_residual<T>(s, n, b, q, r);
+            } else {
+                launch_residual<T>(s, n, b, nullptr, r);
+            }
         } else {
-            launch_residual<T>(s, n, b, nullptr, r);
+            T* xi = static_cast<T*>(d.vec[BV_XI]);
+            launch_fill<T>(s, n, static_cast<T*>(d.vec[BV_P]), T{});
+            if (o.is_init) {
+                launch_fill<T>(s, n, x, T{});
+                launch_fill<T>(s, n, xi, T{});
+                // MatSolverMRTR_ICver.cpp:213-216 subtracts A x only here (x is zero)
+                launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), x, q,
+                               nullptr, nullptr, m.spmv_stage_kb);
+                launch_residual<T>(s, n, b, q, r);
+            } else {
+                launch_copy<T>(s, n, x, xi);
+                launch_residual<T>(s, n, b, nullptr, r);
+            }
         }
         BatchState st{};
         st.normB = normB;
@@ -449,6 +514,8 @@ template <typename T> struct Runner {
         st.save_best = o.is_save_best;
         st.diverge_type = o.diverge_judge_type;
         st.bad_div_thres = o.bad_div_count_thres;
+        st.mrtr_fix = o.mrtr_direction_fix;
+        st.zeta[0] = st.zeta_old[0] = st.nu[0] = 1.0;   // zeta = zeta_old = nu = 1, eta = 0
         B.h_sys[i] = d;
         B.h_state[i] = st;
         sl.state = SLOT_NEW;
@@ -503,13 +570,16 @@ template <typename F> int with_runner(Batch& B, F&& f) {
 
 extern "C" {
 
-int ssgpu_batch_create(ssgpu_context ctx, int dtype, int factor_precision, ssgpu_batch* out) {
+int ssgpu_batch_create(ssgpu_context ctx, int solver, int dtype, int factor_precision, ssgpu_batch* out) {
     if (!ctx || !out || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
     if (factor_precision != SSGPU_FACTOR_F64 && factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
+    if (solver == SSGPU_MRTR_SGS) return SSGPU_UNSUPPORTED;
+    if (solver != SSGPU_ICCG && solver != SSGPU_MRTR_IC) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
     std::unique_ptr<ssgpu_batch_s> h(new ssgpu_batch_s());
     Batch& B = h->b;
     B.c = &ctx->c;
+    B.solver = solver;
     B.dtype = dtype;
     B.precision = factor_precision;
     if (const char* e = std::getenv("SSGPU_SPIN_LIMIT")) B.spin_limit = std::max(0L, std::atol(e));
@@ -584,6 +654,20 @@ int ssgpu_batch_poll(ssgpu_batch bh, int* got, int64_t* tag, ssgpu_result* res) 
     return SSGPU_OK;
 }
 
+int ssgpu_batch_info(ssgpu_batch bh, int* systems, int64_t* iterations_run, int* graph_fallbacks) {
+    if (!bh) return SSGPU_BAD_ARG;
+    Batch& B = bh->b;
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    if (systems) {
+        int k = 0;
+        for (const auto& q : B.slots) k += q.state != SLOT_FREE;
+        *systems = k;
+    }
+    if (iterations_run) *iterations_run = B.iterations_run;
+    if (graph_fallbacks) *graph_fallbacks = B.capture_failures;
+    return SSGPU_OK;
+}
+
 int ssgpu_batch_destroy(ssgpu_batch bh) {
     if (!bh) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
@@ -597,7 +681,7 @@ int ssgpu_batch_destroy(ssgpu_batch bh) {
     SSGPU_GUARD_END
 }
 
-int ssgpu_solve_batch(ssgpu_context ctx, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
+int ssgpu_solve_batch(ssgpu_context ctx, int solver, int nsys, const ssgpu_matrix* mats, const ssgpu_options* opt,
                       double conv_cri, int max_ite, const double* normB, const void* const* d_b, void* const* d_x,
                       ssgpu_result* res) {
     if (!ctx || nsys < 1 || !mats || !opt || !normB || !d_b || !d_x || !res) return SSGPU_BAD_ARG;
@@ -606,7 +690,7 @@ int ssgpu_solve_batch(ssgpu_context ctx, int nsys, const ssgpu_matrix* mats, con
         if (!mats[i] || !d_b[i] || !d_x[i]) return SSGPU_BAD_ARG;
     const int dtype = mats[0]->m.dtype;
     ssgpu_batch bh = nullptr;
-    int st = ssgpu_batch_create(ctx, dtype, opt->factor_precision, &bh);
+    int st = ssgpu_batch_create(ctx, solver, dtype, opt->factor_precision, &bh);
     if (st != SSGPU_OK) return st;
     for (int i = 0; i < nsys && st == SSGPU_OK; ++i)
         st = ssgpu_batch_add(bh, mats[i], opt, conv_cri, max_ite, normB[i], d_b[i], d_x[i], i);

@@ -967,19 +967,19 @@ double finish_residual(Solver& S, int solver, const ssgpu_options& o, bool unsca
     return true_res;
 }
 
-// The end of a batched ICCG solve of one system, as solve_factored ends: the
-// true residual and, for an FP32 factor, the refinement passes (context c's
+// The end of a batched solve of one system, as solve_factored ends: the true
+// residual and, for an FP32 factor, the refinement passes (context c's
 // stream and work space).
 template <typename T>
-double batch_finish(Context* c, Matrix& m, const ssgpu_options& o, bool conv, double conv_cri, int max_ite,
-                    double normB, const T* b, T* x, int& refinements, int& refine_its) {
+double batch_finish(Context* c, Matrix& m, int solver, const ssgpu_options& o, bool conv, double conv_cri,
+                    int max_ite, double normB, const T* b, T* x, int& refinements, int& refine_its) {
     Solver S(m, c, o);
-    return finish_residual<T>(S, SSGPU_ICCG, o, true, conv, conv_cri, max_ite, normB, b, x, refinements, refine_its);
+    return finish_residual<T>(S, solver, o, true, conv, conv_cri, max_ite, normB, b, x, refinements, refine_its);
 }
-template double batch_finish<double>(Context*, Matrix&, const ssgpu_options&, bool, double, int, double,
+template double batch_finish<double>(Context*, Matrix&, int, const ssgpu_options&, bool, double, int, double,
                                      const double*, double*, int&, int&);
-template double batch_finish<zc>(Context*, Matrix&, const ssgpu_options&, bool, double, int, double, const zc*, zc*,
-                                 int&, int&);
+template double batch_finish<zc>(Context*, Matrix&, int, const ssgpu_options&, bool, double, int, double, const zc*,
+                                 zc*, int&, int&);
 
 template <typename T>
 int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite, double normB,

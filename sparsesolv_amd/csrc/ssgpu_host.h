@@ -163,10 +163,10 @@ struct Matrix {
     std::vector<int64_t> vmap;   // stored entry q of P A P^T = caller's entry vmap[q]
 };
 
-// true residual (and FP32-factor refinement) of a finished batched ICCG solve
+// true residual (and FP32-factor refinement) of a finished batched solve
 template <typename T>
-double batch_finish(Context* c, Matrix& m, const ssgpu_options& o, bool conv, double conv_cri, int max_ite,
-                    double normB, const T* b, T* x, int& refinements, int& refine_its);
+double batch_finish(Context* c, Matrix& m, int solver, const ssgpu_options& o, bool conv, double conv_cri,
+                    int max_ite, double normB, const T* b, T* x, int& refinements, int& refine_its);
 
 }  // namespace ssgpu
 

@@ -9,6 +9,9 @@
 
 #include "ssgpu_common.h"
 
+#include <tuple>
+#include <utility>
+
 namespace ssgpu {
 
 constexpr int kBS = 256;          // threads per block of the streaming kernels
@@ -174,7 +177,11 @@ void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta,
 // no-ops).  Every per-system sum runs over the same blocks and the same tree
 // as the one-system kernels above, so a system's iterates are bit-identical to
 // its solo solve.
-enum BatchVec { BV_B = 0, BV_X, BV_P, BV_R, BV_Z, BV_Q, BV_T, BV_BEST, kBatchVecs };
+// per-system vectors: ICCG uses the first eight; IC-MRTR also U, Y, W and its
+// internal iterate XI (its x is copied to X at the end); Z holds MRTR's z and Q
+// its v = A u
+enum BatchVec { BV_B = 0, BV_X, BV_P, BV_R, BV_Z, BV_Q, BV_T, BV_BEST, BV_U, BV_Y, BV_W, BV_XI, kBatchVecs };
+constexpr int kBatchVecsIccg = 8;
 enum BatchStatus {
     BS_ACTIVE = 0,      // iterating
     BS_CONVERGED = 1,   // met the tolerance in the loop
@@ -197,11 +204,13 @@ struct BatchSys {               // device-resident descriptor of one system
     double* part;               // this system's partial sums (max(nblk, rblk) x 2 doubles)
 };
 struct BatchState {             // Krylov scalars and bookkeeping of one system (device)
-    double rz[2], alpha[2], beta[2];   // (r, z), alpha, beta; complex as (re, im)
+    double rz[2], alpha[2], beta[2];   // ICCG: (r, z), alpha, beta; complex as (re, im)
+    double zeta[2], zeta_old[2], eta[2], nu[2], t[2];   // MRTR recurrences
     double normB, normalizer, abs_conv, best, final_res;
     double conv_cri, bad_div_val, normalize_const;
     int max_ite, iterations, bad, status, copy_best;
     int normalize_type, save_best, diverge_type, bad_div_thres;
+    int mrtr_fix;               // MRTR direction: 0 the reference's p = u + t u, 1 p = u + t p
 };
 // vector phases of a batched ICCG iteration and prologue
 enum BatchOp {
@@ -212,7 +221,11 @@ enum BatchOp {
     BO_COPY,          // vec[b] = vec[a]
     BO_SENT,          // vec[a], vec[b] = the sweeps' not-ready sentinel
     BO_CG_UPDATE,     // x += alpha p; r -= alpha q; partial |r|^2; t, z = sentinel
-    BO_PUPDATE        // best = x when asked; p = p*beta + z
+    BO_PUPDATE,       // best = x when asked; p = p*beta + z
+    BO_NEG,           // vec[b] = -1 * vec[a]
+    BO_DOT3,          // partials w.r, w.v, w.y (w.y only after the first iteration), w = vec[W]
+    BO_MRTR_UPDATE,   // p = u + t u (or u + t p); xi += zeta p; y = y eta + zeta v; r -= y; partial |r|^2
+    BO_MRTR_UPDATE2   // best = xi when asked; z = z eta + zeta w; u -= z; t, w = sentinel
 };
 // scalar phases (one workgroup per system: the fixed-tree finalize of its partials, then)
 enum BatchPhase {
@@ -220,27 +233,68 @@ enum BatchPhase {
     BP_RZ0,           // (r, z); PRIMED -> ACTIVE
     BP_ALPHA,         // alpha = rz / (p, Ap)  (partials of the SpMV blocks)
     BP_MONITOR,       // ||r||: convergence, best iterate, divergence, max_ite
-    BP_BETA           // beta = rz' / rz; rz = rz'
+    BP_BETA,          // beta = rz' / rz; rz = rz'
+    BP_ACTIVATE,      // (no sums) PRIMED -> ACTIVE, sweep tickets zeroed
+    BP_MRTR_COEF      // zeta, eta, nu, t from w.r, w.v, w.y (MatSolverMRTR_ICver.cpp:279-300)
 };
-// items: (system, block) pairs; fin: systems, one workgroup each
+// A graph built node by node, without stream capture (so no stream is ever
+// in capture mode, whatever other threads do meanwhile): a batched launcher
+// given a recorder appends a kernel node depending on the previous node
+// instead of launching.
+struct GraphRec {
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t err = hipSuccess;
+};
+// launch kernel(args...) on s, or append it to rec when rec is given
+template <typename... KArgs, size_t... I>
+inline void launch_or_record_impl(GraphRec* rec, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                                  std::tuple<KArgs...>& vals, std::index_sequence<I...>) {
+    void* params[] = {static_cast<void*>(&std::get<I>(vals))..., nullptr};
+    if (!rec) {
+        (void)hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, params, 0, s);
+        return;
+    }
+    if (rec->err != hipSuccess) return;
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(kernel);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = params;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    rec->err = hipGraphAddKernelNode(&node, rec->graph, rec->last ? &rec->last : nullptr, rec->last ? 1 : 0, &p);
+    if (rec->err == hipSuccess) rec->last = node;
+}
+template <typename... KArgs, typename... Args>
+inline void launch_or_record(GraphRec* rec, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t s,
+                             Args... args) {
+    std::tuple<KArgs...> vals(args...);
+    launch_or_record_impl(rec, kernel, grid, block, s, vals, std::index_sequence_for<KArgs...>{});
+}
+
+// items: (system, block) pairs; fin: systems, one workgroup each; rec: append
+// to a graph instead of launching
 template <typename T>
 void launch_batch_spmv(hipStream_t s, int nitems, const int2* items, const BatchSys* sys, const BatchState* st,
-                       int want, int in, int out, int with_dot);
+                       int want, int in, int out, int with_dot, GraphRec* rec = nullptr);
+// tickets (may be null): zeroed first by the launch's first workgroup
 template <typename T>
 void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, const BatchSys* sys, BatchState* st,
-                      int want, int a, int b, int c);
+                      int want, int a, int b, int c, int* tickets = nullptr, GraphRec* rec = nullptr);
 // active: count of systems still iterating (decremented as they finish);
-// tickets: the sweep tickets, zeroed by the PROLOGUE and MONITOR phases
+// tickets: the sweep tickets, zeroed by the PROLOGUE, ACTIVATE and MONITOR phases
 template <typename T>
 void launch_batch_finalize(hipStream_t s, int phase, int nfin, const int* fin, const BatchSys* sys, BatchState* st,
-                           int want, int* active, int* tickets);
+                           int want, int* active, int* tickets, GraphRec* rec = nullptr);
 // persistent sync-free sweep over the chunks of every system in `order`
 // (system, chunk) ticket order; mode FWD_DIV_LAST (vec[in] -> vec[out]) or
 // BWD_IC (aux vec[in] -> vec[out]); F the factor's storage type
 template <typename T, typename F>
 void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder, const int2* order, int* ticket,
                         int* err, long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in,
-                        int out, int max_idle);
+                        int out, int max_idle, GraphRec* rec = nullptr);
 // dst[e] = map[e] >= 0 ? src[map[e]] : 0 (narrowing to F)
 template <typename T, typename F>
 void launch_gather_pad(hipStream_t s, int64_t n, const int* map, const T* src, F* dst);

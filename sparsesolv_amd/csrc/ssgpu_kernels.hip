@@ -492,7 +492,9 @@ __global__ __launch_bounds__(kBS) void k_batch_spmv(const int2* __restrict__ ite
 
 template <typename T, int OP>
 __global__ __launch_bounds__(kBS) void k_batch_vec(const int2* __restrict__ items, const BatchSys* __restrict__ sys,
-                                                   const BatchState* __restrict__ st, int want, int a, int b, int c) {
+                                                   const BatchState* __restrict__ st, int want, int a, int b, int c,
+                                                   int* __restrict__ tickets) {
+    if (tickets && blockIdx.x == 0 && threadIdx.x < 2) tickets[threadIdx.x] = 0;
     const int2 it = items[blockIdx.x];
     const BatchState& q = st[it.x];
     if (q.status != want) return;
@@ -567,6 +569,71 @@ __global__ __launch_bounds__(kBS) void k_batch_vec(const int2* __restrict__ item
             for (int i = i0; i < n; i += stride) best[i] = x[i];
         }
         for (int i = i0; i < n; i += stride) p[i] = Num<T>::add(Num<T>::mul(p[i], beta), z[i]);
+    } else if (OP == BO_NEG) {
+        const T* u = bvec<T>(S, a);
+        T* v = bvec<T>(S, b);
+        for (int i = i0; i < n; i += stride) v[i] = neg(u[i]);
+    } else if (OP == BO_DOT3) {
+        // as k_dot3 (w; r, v, y), y from the second iteration on (MatSolverMRTR_ICver.cpp:276-281)
+        const T* w = bvec<T>(S, BV_W);
+        const T* r = bvec<T>(S, BV_R);
+        const T* v = bvec<T>(S, BV_Q);
+        const T* y = q.iterations > 0 ? bvec<T>(S, BV_Y) : nullptr;
+        double acc[3 * K] = {};
+        for (int i = i0; i < n; i += stride) {
+            const T wi = w[i];
+            Parts<T>::put(acc, Num<T>::mul(wi, r[i]));
+            Parts<T>::put(acc + K, Num<T>::mul(wi, v[i]));
+            if (y) Parts<T>::put(acc + 2 * K, Num<T>::mul(wi, y[i]));
+        }
+        block_sum<3 * K>(acc);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3 * K; ++k) S.part[it.y * 3 * K + k] = acc[k];
+    } else if (OP == BO_MRTR_UPDATE) {
+        // as k_mrtr_update on the internal iterate xi
+        const T tt = from_parts<T>(q.t), zeta = from_parts<T>(q.zeta), eta = from_parts<T>(q.eta);
+        const int fix = q.mrtr_fix;
+        const T* u = bvec<T>(S, BV_U);
+        const T* v = bvec<T>(S, BV_Q);
+        T* p = bvec<T>(S, BV_P);
+        T* x = bvec<T>(S, BV_XI);
+        T* y = bvec<T>(S, BV_Y);
+        T* r = bvec<T>(S, BV_R);
+        double acc[1] = {0.0};
+        for (int i = i0; i < n; i += stride) {
+            const T ui = u[i];
+            const T pi = fix ? Num<T>::add(ui, Num<T>::mul(tt, p[i])) : Num<T>::add(ui, Num<T>::mul(tt, ui));
+            p[i] = pi;
+            x[i] = Num<T>::add(x[i], Num<T>::mul(zeta, pi));
+            const T yi = Num<T>::add(Num<T>::mul(y[i], eta), Num<T>::mul(zeta, v[i]));
+            y[i] = yi;
+            const T ri = Num<T>::sub(r[i], yi);
+            r[i] = ri;
+            acc[0] += Num<T>::abs2(ri);
+        }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) S.part[it.y] = acc[0];
+    } else if (OP == BO_MRTR_UPDATE2) {
+        // the best-iterate copy the monitor asked for, then k_mrtr_update2; the
+        // sweeps' outputs t and w go back to the sentinel once w is read
+        const T zeta = from_parts<T>(q.zeta), eta = from_parts<T>(q.eta);
+        if (q.copy_best) {
+            const T* x = bvec<T>(S, BV_XI);
+            T* best = bvec<T>(S, BV_BEST);
+            for (int i = i0; i < n; i += stride) best[i] = x[i];
+        }
+        const T sv = sentinel_value<T>();
+        T* w = bvec<T>(S, BV_W);
+        T* z = bvec<T>(S, BV_Z);
+        T* u = bvec<T>(S, BV_U);
+        T* t = bvec<T>(S, BV_T);
+        for (int i = i0; i < n; i += stride) {
+            const T zi = Num<T>::add(Num<T>::mul(z[i], eta), Num<T>::mul(zeta, w[i]));
+            z[i] = zi;
+            u[i] = Num<T>::sub(u[i], zi);
+            w[i] = sv;
+            t[i] = sv;
+        }
     }
 }
 
@@ -579,13 +646,15 @@ template <typename T, int PH>
 __global__ __launch_bounds__(kBS) void k_batch_finalize(const int* __restrict__ fin, const BatchSys* __restrict__ sys,
                                                         BatchState* __restrict__ st, int want, int* __restrict__ active,
                                                         int* __restrict__ tickets) {
-    if ((PH == BP_PROLOGUE || PH == BP_MONITOR) && blockIdx.x == 0 && threadIdx.x < 2) tickets[threadIdx.x] = 0;
+    if ((PH == BP_PROLOGUE || PH == BP_MONITOR || PH == BP_ACTIVATE) && blockIdx.x == 0 && threadIdx.x < 2)
+        tickets[threadIdx.x] = 0;
     const int s = fin[blockIdx.x];
     if (st[s].status != want) return;
     const BatchSys& S = sys[s];
-    constexpr int KP = (PH == BP_PROLOGUE || PH == BP_MONITOR) ? 1 : Parts<T>::K;
-    __shared__ double sum[2];
-    finalize_parts(PH == BP_ALPHA ? S.nblk : S.rblk, KP, S.part, sum);
+    constexpr int K = Parts<T>::K;
+    constexpr int KP = (PH == BP_PROLOGUE || PH == BP_MONITOR) ? 1 : PH == BP_MRTR_COEF ? 3 * K : K;
+    __shared__ double sum[6];
+    if (PH != BP_ACTIVATE) finalize_parts(PH == BP_ALPHA ? S.nblk : S.rblk, KP, S.part, sum);
     if (threadIdx.x != 0) return;
     BatchState& q = st[s];
     constexpr double kSmallAbs = 1.0e-20;
@@ -623,6 +692,36 @@ __global__ __launch_bounds__(kBS) void k_batch_finalize(const int* __restrict__ 
         const T rz1 = from_parts<T>(sum);
         to_parts(Num<T>::div(rz1, from_parts<T>(q.rz)), q.beta);
         to_parts(rz1, q.rz);
+    } else if (PH == BP_ACTIVATE) {
+        if (q.max_ite <= 0) {
+            q.status = BS_STOPPED;
+            atomicSub(active, 1);
+        } else {
+            q.status = BS_ACTIVE;
+        }
+    } else if (PH == BP_MRTR_COEF) {
+        // the scalar recurrence of MatSolverMRTR_ICver.cpp:279-300, in the
+        // one-system driver's operation order
+        const T w_r = from_parts<T>(sum), v_w = from_parts<T>(sum + K);
+        T zeta_old = from_parts<T>(q.zeta_old), nu = from_parts<T>(q.nu), zeta, eta;
+        if (q.iterations == 0) {
+            zeta = Num<T>::div(w_r, v_w);
+            zeta_old = zeta;
+            eta = Num<T>::zero();
+        } else {
+            const T w_y = from_parts<T>(sum + 2 * K);
+            const T t0 = Num<T>::div(from_real<T>(1.0), Num<T>::sub(Num<T>::mul(nu, v_w), Num<T>::mul(w_y, w_y)));
+            zeta = Num<T>::mul(Num<T>::mul(nu, w_r), t0);
+            eta = Num<T>::mul(Num<T>::mul(neg(w_y), w_r), t0);
+        }
+        nu = Num<T>::mul(zeta, w_r);
+        const T tt = Num<T>::div(Num<T>::mul(eta, zeta_old), zeta);
+        zeta_old = zeta;
+        to_parts(zeta, q.zeta);
+        to_parts(zeta_old, q.zeta_old);
+        to_parts(eta, q.eta);
+        to_parts(nu, q.nu);
+        to_parts(tt, q.t);
     } else {   // BP_MONITOR
         const double pn = sqrt(sum[0]);
         q.iterations += 1;
@@ -682,19 +781,24 @@ void launch_spmv(hipStream_t s, int nblk, const int* blk, const int* rp, const i
 
 template <typename T>
 void launch_batch_spmv(hipStream_t s, int nitems, const int2* items, const BatchSys* sys, const BatchState* st,
-                       int want, int in, int out, int with_dot) {
+                       int want, int in, int out, int with_dot, GraphRec* rec) {
     if (nitems <= 0) return;
-    hipLaunchKernelGGL((k_batch_spmv<T, 16384>), dim3(nitems), dim3(kBS), 0, s, items, sys, st, want, in, out,
-                       with_dot);
+    launch_or_record(rec, k_batch_spmv<T, 16384>, dim3(nitems), dim3(kBS), s, items, sys, st, want, in, out,
+                     with_dot);
 }
 
 template <typename T>
 void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, const BatchSys* sys, BatchState* st,
-                      int want, int a, int b, int c) {
+                      int want, int a, int b, int c, int* tickets, GraphRec* rec) {
     if (nitems <= 0) return;
+    const BatchState* cst = st;
 #define SSGPU_BV(OP) \
-    hipLaunchKernelGGL((k_batch_vec<T, OP>), dim3(nitems), dim3(kBS), 0, s, items, sys, st, want, a, b, c)
+    launch_or_record(rec, k_batch_vec<T, OP>, dim3(nitems), dim3(kBS), s, items, sys, cst, want, a, b, c, tickets)
     switch (op) {
+        case BO_NEG: SSGPU_BV(BO_NEG); break;
+        case BO_DOT3: SSGPU_BV(BO_DOT3); break;
+        case BO_MRTR_UPDATE: SSGPU_BV(BO_MRTR_UPDATE); break;
+        case BO_MRTR_UPDATE2: SSGPU_BV(BO_MRTR_UPDATE2); break;
         case BO_FILL0: SSGPU_BV(BO_FILL0); break;
         case BO_RESID: SSGPU_BV(BO_RESID); break;
         case BO_NORM2: SSGPU_BV(BO_NORM2); break;
@@ -709,15 +813,17 @@ void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, cons
 
 template <typename T>
 void launch_batch_finalize(hipStream_t s, int phase, int nfin, const int* fin, const BatchSys* sys, BatchState* st,
-                           int want, int* active, int* tickets) {
+                           int want, int* active, int* tickets, GraphRec* rec) {
     if (nfin <= 0) return;
 #define SSGPU_BF(PH) \
-    hipLaunchKernelGGL((k_batch_finalize<T, PH>), dim3(nfin), dim3(kBS), 0, s, fin, sys, st, want, active, tickets)
+    launch_or_record(rec, k_batch_finalize<T, PH>, dim3(nfin), dim3(kBS), s, fin, sys, st, want, active, tickets)
     switch (phase) {
         case BP_PROLOGUE: SSGPU_BF(BP_PROLOGUE); break;
         case BP_RZ0: SSGPU_BF(BP_RZ0); break;
         case BP_ALPHA: SSGPU_BF(BP_ALPHA); break;
         case BP_BETA: SSGPU_BF(BP_BETA); break;
+        case BP_ACTIVATE: SSGPU_BF(BP_ACTIVATE); break;
+        case BP_MRTR_COEF: SSGPU_BF(BP_MRTR_COEF); break;
         default: SSGPU_BF(BP_MONITOR); break;
     }
 #undef SSGPU_BF
@@ -852,11 +958,11 @@ SSGPU_INST(double)
 SSGPU_INST(zc)
 #define SSGPU_BINST(T)                                                                                      \
     template void launch_batch_spmv<T>(hipStream_t, int, const int2*, const BatchSys*, const BatchState*, int, int, \
-                                       int, int);                                                               \
+                                       int, int, GraphRec*);                                                    \
     template void launch_batch_vec<T>(hipStream_t, int, int, const int2*, const BatchSys*, BatchState*, int, int, \
-                                      int, int);                                                                \
+                                      int, int, int*, GraphRec*);                                               \
     template void launch_batch_finalize<T>(hipStream_t, int, int, const int*, const BatchSys*, BatchState*, int, \
-                                           int*, int*);
+                                           int*, int*, GraphRec*);
 SSGPU_BINST(double)
 SSGPU_BINST(zc)
 template void launch_sweep<double, double>(hipStream_t, int, int, int, const int*, const int*, const double*,

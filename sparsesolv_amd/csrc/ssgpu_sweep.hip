@@ -447,10 +447,10 @@ void sweep_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks
 template <typename T, typename F, int MODE>
 void batch_sweep_kmax(hipStream_t s, int kmax, int g, int norder, const int2* order, int* ticket, int* err,
                       long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in, int out,
-                      int max_idle) {
-#define SSGPU_BS(K)                                                                                            \
-    hipLaunchKernelGGL((k_batch_sweep<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, norder, order, ticket, err, \
-                       spin_limit, sys, st, want, in, out, max_idle)
+                      int max_idle, GraphRec* rec) {
+#define SSGPU_BS(K)                                                                                                \
+    launch_or_record(rec, k_batch_sweep<T, F, MODE, K>, dim3(g), dim3(kSfRows), s, norder, order, ticket, err, \
+                     spin_limit, sys, st, want, in, out, max_idle)
     if (kmax <= 8) SSGPU_BS(8);
     else if (kmax <= 12) SSGPU_BS(12);
     else if (kmax <= 16) SSGPU_BS(16);
@@ -569,15 +569,15 @@ void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int 
 template <typename T, typename F>
 void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder, const int2* order, int* ticket,
                         int* err, long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in,
-                        int out, int max_idle) {
+                        int out, int max_idle, GraphRec* rec) {
     if (norder <= 0) return;
     const int g = std::max(1, std::min(grid, norder));
     if (mode == FWD_DIV_LAST)
         batch_sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in,
-                                             out, max_idle);
+                                             out, max_idle, rec);
     else
         batch_sweep_kmax<T, F, BWD_IC>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in, out,
-                                       max_idle);
+                                       max_idle, rec);
 }
 
 template <typename T>
@@ -630,7 +630,7 @@ template void launch_sweep_levels<zc, fc>(hipStream_t, int, int, int, const int*
                                              const zc*, zc*);
 #define SSGPU_BSI(T, F)                                                                                   \
     template void launch_batch_sweep<T, F>(hipStream_t, int, int, int, int, const int2*, int*, int*, long,       \
-                                           const BatchSys*, const BatchState*, int, int, int, int);
+                                           const BatchSys*, const BatchState*, int, int, int, int, GraphRec*);
 SSGPU_BSI(double, double)
 SSGPU_BSI(double, float)
 SSGPU_BSI(zc, zc)

@@ -124,6 +124,8 @@ class _Runner:
         self.closing = False
         self.ctx = None
         self.occupancy = []          # systems in the batch at each check
+        self.failures = []           # runner failures outside the solves (each restarted the session)
+        self.graph_fallbacks = 0     # graph captures that fell back to eager launches
         self.thread = threading.Thread(target=self._run, daemon=True)
         self.thread.start()
 
@@ -150,56 +152,83 @@ class _Runner:
         tag = 0
         try:
             while True:
-                with self.cv:
-                    while not self.inbox and not inflight and not self.closing:
-                        self.cv.wait()
-                    if self.closing and not self.inbox and not inflight:
-                        break
-                    items = list(self.inbox)
-                    self.inbox.clear()
-                keep = []
-                for item in items:
-                    r, fut, key, e, hit = item
-                    n = e.matrix.n
-                    # a new system's vectors must fit the budget too, unless the batch is empty
-                    if inflight and not eng._room(_batch_vector_bytes(n, self.complex)):
-                        keep.append(item)
-                        continue
-                    b = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
-                    x = np.zeros(n, e.matrix.dtype)
-                    opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
-                    try:
-                        batch.add_host(e.matrix, b, x, tag, normB=float(np.sqrt(np.sum(np.abs(b) ** 2))),
-                                       conv_cri=r.conv_cri, max_ite=r.max_ite, opts=opts)
-                    except capi.SolverError as ex:
-                        eng._respond(fut, r, None, None, hit, str(ex))
-                        eng._release(key, e)
-                        continue
-                    inflight[tag] = (r, fut, key, e, hit, b, x)
-                    tag += 1
-                if keep:
-                    with self.cv:
-                        self.inbox.extendleft(reversed(keep))
-                if not inflight:
-                    continue
-                err = None
                 try:
-                    running, _ = batch.step(eng.check_every)
-                    self.occupancy.append(running)
-                    eng._note_resident()
-                except capi.SolverError as ex:
-                    err = str(ex)
-                for t, res in batch.poll():
-                    r, fut, key, e, hit, b, x = inflight.pop(t)
-                    if res.status in (capi.OK, capi.NOT_CONVERGED):
-                        eng._respond(fut, r, x, res, hit, None)
-                    else:
-                        eng._respond(fut, r, None, res, hit, err or f"batched solve failed (status {res.status})")
-                    eng._release(key, e)
+                    tag, stop = self._turn(batch, inflight, tag)
+                except Exception as ex:   # noqa: BLE001 - every waiting request hears of it
+                    # a failure outside the solves themselves: answer every request of
+                    # the batch and the inbox with it, and start a fresh session
+                    self.failures.append(str(ex))
+                    with self.cv:
+                        items = list(self.inbox)
+                        self.inbox.clear()
+                    for r, fut, key, e, hit, _b, _x in inflight.values():
+                        eng._respond(fut, r, None, None, hit, f"batch runner failed: {ex}")
+                        eng._release(key, e)
+                    for r, fut, key, e, hit in items:
+                        eng._respond(fut, r, None, None, hit, f"batch runner failed: {ex}")
+                        eng._release(key, e)
+                    inflight.clear()
+                    batch.close()
+                    batch = capi.Batch(self.ctx, self.complex, self.precision)
+                    stop = False
+                if stop:
+                    break
         finally:
+            self.graph_fallbacks += batch.info()[2]
             batch.close()
             self.ctx.close()
             self.ctx = None
+
+    def _turn(self, batch, inflight, tag):
+        """Admit what the inbox holds, run one step, answer the finished; (next tag, stop)."""
+        eng = self.eng
+        with self.cv:
+            while not self.inbox and not inflight and not self.closing:
+                self.cv.wait()
+            if self.closing and not self.inbox and not inflight:
+                return tag, True
+            items = list(self.inbox)
+            self.inbox.clear()
+        keep = []
+        for item in items:
+            r, fut, key, e, hit = item
+            n = e.matrix.n
+            # a new system's vectors must fit the budget too, unless the batch is empty
+            if inflight and not eng._room(_batch_vector_bytes(n, self.complex)):
+                keep.append(item)
+                continue
+            b = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
+            x = np.zeros(n, e.matrix.dtype)
+            opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
+            try:
+                batch.add_host(e.matrix, b, x, tag, normB=float(np.sqrt(np.sum(np.abs(b) ** 2))),
+                               conv_cri=r.conv_cri, max_ite=r.max_ite, opts=opts)
+            except capi.SolverError as ex:
+                eng._respond(fut, r, None, None, hit, str(ex))
+                eng._release(key, e)
+                continue
+            inflight[tag] = (r, fut, key, e, hit, b, x)
+            tag += 1
+        if keep:
+            with self.cv:
+                self.inbox.extendleft(reversed(keep))
+        if not inflight:
+            return tag, False
+        err = None
+        try:
+            running, _ = batch.step(eng.check_every)
+            self.occupancy.append(running)
+            eng._note_resident()
+        except capi.SolverError as ex:
+            err = str(ex)
+        for t, res in batch.poll():
+            r, fut, key, e, hit, b, x = inflight.pop(t)
+            if res.status in (capi.OK, capi.NOT_CONVERGED):
+                eng._respond(fut, r, x, res, hit, None)
+            else:
+                eng._respond(fut, r, None, res, hit, err or f"batched solve failed (status {res.status})")
+            eng._release(key, e)
+        return tag, False
 
 
 class Engine:
@@ -257,6 +286,15 @@ class Engine:
     def batches(self):
         """Systems in the batch at each check of every runner."""
         return [k for r in list(self.runners.values()) for k in r.occupancy]
+
+    @property
+    def runner_failures(self):
+        return [f for r in list(self.runners.values()) for f in r.failures]
+
+    @property
+    def graph_fallbacks(self):
+        """Graph captures of closed runners that fell back to eager launches."""
+        return sum(r.graph_fallbacks for r in list(self.runners.values()))
 
     def close(self):
         with self.qcv:

@@ -1,4 +1,4 @@
-"""Batched ICCG / COCG (ssgpu_batch_*, ssgpu_solve_batch) through the C-ABI.
+"""Batched ICCG / COCG and IC-MRTR (ssgpu_batch_*, ssgpu_solve_batch) through the C-ABI.
 
 Many independent systems of uneven size, and several right-hand sides of one
 matrix, advance together: one launch per iteration phase for all of them,
@@ -72,18 +72,18 @@ class _Set:
             m.close()
 
 
-def _solo(torch, ctx, m, d_b, x0, normB, opts, max_ite):
+def _solo(torch, ctx, m, d_b, x0, normB, opts, max_ite, solver=capi.ICCG):
     d_x = torch.from_numpy(np.ascontiguousarray(x0)).cuda()
     torch.cuda.synchronize()
-    r = m.solve_factored(ctx, capi.ICCG, d_b.data_ptr(), d_x.data_ptr(), conv_cri=CONV, max_ite=max_ite,
+    r = m.solve_factored(ctx, solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=CONV, max_ite=max_ite,
                          normB=normB, opts=opts)
     ctx.synchronize()
     return d_x.cpu().numpy(), r
 
 
-def _same_as_solo(torch, ctx, st, d_b, x_b, r_b, x0, normB, opts, max_ite, label):
+def _same_as_solo(torch, ctx, st, d_b, x_b, r_b, x0, normB, opts, max_ite, label, solver=capi.ICCG):
     s, m = st
-    x_s, r_s = _solo(torch, ctx, m, d_b, x0, normB, opts, max_ite)
+    x_s, r_s = _solo(torch, ctx, m, d_b, x0, normB, opts, max_ite, solver)
     assert r_b.converged == r_s.converged, label
     assert r_b.iterations == r_s.iterations, (label, r_b.iterations, r_s.iterations)
     assert r_b.final_residual == r_s.final_residual, label
@@ -299,5 +299,72 @@ def test_batch_sweep_spin_limit_then_recovery(torch_cuda, ctx, monkeypatch):
         for k in range(len(S.items)):
             assert res[k].iterations == ref[k].iterations
             assert np.array_equal(d_x[k].cpu().numpy(), x_ref[k])
+    finally:
+        S.close()
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+@pytest.mark.parametrize("fix", [0, 1], ids=["ref_direction", "std_direction"])
+def test_batched_mrtr_bit_identical_to_solo(torch_cuda, ctx, cplx, fix):
+    """IC-MRTR batched (MatSolverMRTR_ICver.cpp:183-373, with the reference's
+    direction p = u + t u or the standard one): every system as its solo
+    solve, including the FP32 factor with its refinement pass (standard
+    direction) and per-system options."""
+    torch = torch_cuda
+    systems = _systems(cplx)[:5]
+    precision = capi.FACTOR_F32 if fix else capi.FACTOR_F64
+    S = _Set(torch, ctx, systems, precision)
+    try:
+        rng = np.random.default_rng(8)
+        cases = [dict(), dict(is_save_best=1), dict(conv_normalize_type=1), dict(is_init=0), dict()]
+        x0s = [1e-3 * rng.standard_normal(s.n) * (1 + 1j if cplx else 1) if "is_init" in c else None
+               for c, (s, _) in zip(cases, S.items)]
+        d_b, d_x = S.vectors(x0s)
+        normB = [float(np.linalg.norm(s.rhs)) for s, _ in S.items]
+        B = capi.Batch(ctx, cplx, precision, solver=capi.MRTR_IC)
+        got = {}
+        try:
+            for k, (st, c) in enumerate(zip(S.items, cases)):
+                o = capi.default_options(factor_precision=precision, mrtr_direction_fix=fix, **c)
+                B.add(st[1], d_b[k].data_ptr(), d_x[k].data_ptr(), k, normB=normB[k], conv_cri=CONV,
+                      max_ite=3000 if k != 1 else 5, opts=o)
+            while len(got) < len(S.items):
+                B.step(6)
+                got.update(dict(B.poll()))
+        finally:
+            B.close()
+        c2 = capi.Context(0)
+        try:
+            for k, (st, c) in enumerate(zip(S.items, cases)):
+                o = capi.default_options(factor_precision=precision, mrtr_direction_fix=fix, **c)
+                x0 = np.zeros_like(st[0].rhs) if x0s[k] is None else x0s[k]
+                _same_as_solo(torch, c2, st, d_b[k], d_x[k].cpu().numpy(), got[k], x0, normB[k], o,
+                              3000 if k != 1 else 5, k, solver=capi.MRTR_IC)
+        finally:
+            c2.close()
+        print(f"batched MRTR cplx={cplx} fix={fix}: iterations {[got[k].iterations for k in sorted(got)]}",
+              flush=True)
+    finally:
+        S.close()
+
+
+def test_batched_mrtr_tree_order_oracle(torch_cuda, ctx):
+    """A batched IC-MRTR system bit-identical to the oracle's IC-MRTR in the
+    GPU's reduction order."""
+    torch = torch_cuda
+    sel = _systems(False)[:2]
+    S = _Set(torch, ctx, sel)
+    try:
+        d_b, d_x = S.vectors()
+        normB = [O.norm(s.rhs) for s, _ in S.items]
+        res = capi.solve_batch(ctx, [m for _, m in S.items], [t.data_ptr() for t in d_b],
+                               [t.data_ptr() for t in d_x], normB, conv_cri=CONV, max_ite=3000, solver=capi.MRTR_IC)
+        for k, ((s, a), r) in enumerate(zip(sel, res)):
+            with O.reduction_order(1):
+                c_o, x_o, log_o = O.solve("mrtr", s.row_ptr, s.col_idx, s.values, s.rhs, conv_cri=CONV, max_ite=3000,
+                                          normB=normB[k], accel_factor=a, precondition_type=0)
+            assert r.converged == c_o
+            assert r.iterations == len(log_o) - 1
+            assert np.array_equal(d_x[k].cpu().numpy(), x_o), k
     finally:
         S.close()

@@ -221,17 +221,25 @@ def _ref_or_oracle(solver, s, normB, **kw):
 def test_full_size_solvers_against_reference(torch_cuda, big, case):
     """The n=95 system through every solver (and ICCG at both ends of the
     1.05-1.3 shift sweep) against the compiled reference: same convergence,
-    iterations within +-2%, first residuals to 1e-10, x within 1e-8.  The
-    solvers whose returned x is the solution of the recurrence must also meet
-    the tolerance in true residual (IC-MRTR in the reference's form returns
-    p = u + t*u's x, MatSolverMRTR_ICver.cpp:302-304, which is not)."""
+    iterations within +-2%, first residuals to 1e-10, x within 1e-8.
+
+    IC-MRTR in the reference's form returns the x of p = u + t*u
+    (MatSolverMRTR_ICver.cpp:302-304), not the iterate of the recurrence: that
+    x moves by about 1e-6 at this size when only the order of the dot products
+    changes, in the reference itself.  For it the parity proof is the oracle
+    run in the GPU's reduction order (bit-identical x and log), and x is held
+    to the reference's own reordering sensitivity.  ICCG's x must meet the
+    tolerance in true residual; SGS-MRTR converges on the preconditioned,
+    scaled residual ||C^-1 r~|| / ||b~|| (MatSolverMRTR_SGSver.cpp:131-134, 227-237), so
+    its true residual is reported and bounded only loosely."""
     name, code, pt, accel = case
     s, m = big
     normB = float(np.linalg.norm(s.rhs))
     o = capi.default_options(is_save_log=1, accel_factor=accel)
     conv, x, log, res = m.solve(code, s.rhs, conv_cri=1e-8, max_ite=20000, normB=normB, opts=o)
-    c_r, x_r, log_r = _ref_or_oracle("iccg" if name == "iccg" else "mrtr", s, normB, conv_cri=1e-8,
-                                     max_ite=20000, accel_factor=accel, precondition_type=pt)
+    solver = "iccg" if name == "iccg" else "mrtr"
+    c_r, x_r, log_r = _ref_or_oracle(solver, s, normB, conv_cri=1e-8, max_ite=20000, accel_factor=accel,
+                                     precondition_type=pt)
     it_g, it_r = len(log) - 1, len(log_r) - 1
     rel = float(np.linalg.norm(x - x_r) / np.linalg.norm(x_r))
     true_res = float(np.linalg.norm(s.rhs - s.scipy() @ x) / normB)
@@ -240,9 +248,21 @@ def test_full_size_solvers_against_reference(torch_cuda, big, case):
     assert conv == c_r
     assert abs(it_g - it_r) <= max(1, int(0.02 * it_r)), (it_g, it_r)
     np.testing.assert_allclose(log[:6], log_r[:6], rtol=1e-10)
-    assert rel <= 1e-8, rel
-    if name != "mrtr_ic":
+    xtol = 1e-8
+    if name == "mrtr_ic":
+        print("  running the oracle in the GPU's reduction order", flush=True)
+        with O.reduction_order(1):
+            c_t, x_t, log_t = O.solve(solver, s.row_ptr, s.col_idx, s.values, s.rhs, conv_cri=1e-8, max_ite=20000,
+                                      normB=normB, accel_factor=accel, precondition_type=pt)
+        assert c_t == conv and np.array_equal(log, log_t) and np.array_equal(x, x_t)
+        sens = float(np.linalg.norm(x_t - x_r) / np.linalg.norm(x_r))
+        print(f"  bit-identical to the tree-order oracle; reference reorder sensitivity {sens:.3e}", flush=True)
+        xtol = max(xtol, sens)
+    assert rel <= xtol, rel
+    if name == "iccg":
         assert true_res <= 1.5e-8
+    elif name == "mrtr_sgs":
+        assert true_res <= 1e-6
 
 
 # ---------------------------------------------------------------- tree order at full size
