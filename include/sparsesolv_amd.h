@@ -165,6 +165,24 @@ int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix m, int solver, const ss
                          double conv_cri, int max_ite, double normB, const void* d_b, void* d_x,
                          double* log, int log_cap, ssgpu_result* res);
 
+/* Allocation-free form of ssgpu_solve_factored, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and the same results, bit for bit; the solve's work vectors,
+ * reduction partials, scalar block and sweep tickets come from a fixed device
+ * array the library holds on each GPU (256 MiB, enough for about 4 million
+ * real or 2 million complex unknowns in ICCG, fewer for the MRTR solvers),
+ * and the call makes no device allocation (it needs a factor built by
+ * ssgpu_factorize and a context created beforehand).  A system whose work
+ * space does not fit returns SSGPU_BAD_ARG with the sizes in
+ * ssgpu_last_error(); ABMC-ordered matrices return SSGPU_UNSUPPORTED.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from different threads or contexts) race and must be serialized by the
+ * caller.  iterate_ms is host wall time. */
+int ssgpu_solve_factored_noalloc(ssgpu_context ctx, ssgpu_matrix m, int solver, const ssgpu_options* opt,
+                                 double conv_cri, int max_ite, double normB, const void* d_b, void* d_x,
+                                 double* log, int log_cap, ssgpu_result* res);
+
 /* Block ICCG / COCG: nrhs (1..32) right-hand sides against the factor of
  * ssgpu_factorize in one pass of kernels per iteration (the sweeps carry
  * every column through the dependency chain at once).  d_B and d_X hold

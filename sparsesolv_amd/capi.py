@@ -79,6 +79,7 @@ def lib():
         L.ssgpu_solve_device.argtypes = [V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
         L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
+        L.ssgpu_solve_factored_noalloc.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_solve_block.argtypes = [V, V, P(Options), D, I, I, V, V, V, P(Result)]
         L.ssgpu_batch_create.argtypes = [V, I, I, I, P(V)]
         L.ssgpu_batch_add.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
@@ -243,6 +244,18 @@ class Matrix:
         cap = 0 if log is None else len(log)
         st = lib().ssgpu_solve_factored(ctx.h, self.h, solver, ctypes.byref(o), conv_cri, max_ite, normB, d_b, d_x,
                                         None if log is None else log.ctypes.data, cap, ctypes.byref(res))
+        check(st, allow=(OK, NOT_CONVERGED))
+        return res
+
+    def solve_factored_noalloc(self, ctx: "Context", solver, d_b: int, d_x: int, *, conv_cri=1e-8, max_ite=10000,
+                               normB=0.0, opts=None, log=None):
+        """solve_factored with every device buffer taken from the library's fixed scratch array (no device
+        allocation; calls on one GPU must not overlap)."""
+        o = opts or default_options()
+        res = Result()
+        cap = 0 if log is None else len(log)
+        st = lib().ssgpu_solve_factored_noalloc(ctx.h, self.h, solver, ctypes.byref(o), conv_cri, max_ite, normB, d_b,
+                                                d_x, None if log is None else log.ctypes.data, cap, ctypes.byref(res))
         check(st, allow=(OK, NOT_CONVERGED))
         return res
 

@@ -381,11 +381,25 @@ inline double hdiv(double a, double b) { return a / b; }
 
 template <typename T> constexpr int kparts() { return sizeof(T) / sizeof(double); }
 
+// Device work space of a solve carved from caller-supplied memory instead of
+// the context's pools (the allocation-free entry point): vector slots of n
+// values, the reduction partials, the scalar block (whose last slot is the
+// sweep error flag) and the sweep tickets.
+struct Workspace {
+    char* vecs = nullptr;          // nslots vectors of vec_bytes each
+    size_t vec_bytes = 0;
+    int slot_of[16];               // work-vector index -> slot (-1: not planned for this solve)
+    double* partial = nullptr;
+    double* scal = nullptr;        // kScalSlots doubles
+    int* tickets = nullptr;        // 16 ints
+};
+
 struct Solver {
     Matrix& m;
     Context* c;
     hipStream_t s;
     const ssgpu_options& o;
+    Workspace* ws = nullptr;   // set: every device buffer of the solve comes from it
     long spin_limit = kSweepSpinLimit;   // polling passes without progress before a sweep wave gives up
     std::vector<double> log;
     int iterations = 0;
@@ -399,6 +413,10 @@ struct Solver {
     Solver(Matrix& mm, const ssgpu_options& oo) : Solver(mm, mm.ctx, oo) {}
 
     template <typename T> T* vec(size_t idx) {
+        if (ws) {
+            if (idx >= 16 || ws->slot_of[idx] < 0) throw Error{HIP_ERROR, "work vector outside the planned work space"};
+            return reinterpret_cast<T*>(ws->vecs + ws->vec_bytes * (size_t)ws->slot_of[idx]);
+        }
         while (c->work.size() <= idx) c->work.emplace_back(new DevBuf());
         DevBuf& b = *c->work[idx];
         const size_t need = sizeof(T) * (size_t)std::max(m.n, 1);
@@ -406,14 +424,29 @@ struct Solver {
         return b.as<T>();
     }
     double* partial() {
+        if (ws) return ws->partial;
         const size_t need = sizeof(double) * 8 * (size_t)std::max(m.nblk, kMaxReduceBlocks);
         if (c->partial.bytes < need) c->partial.alloc(c, need);
         return c->partial.as<double>();
     }
+    double* scal() { return ws ? ws->scal : scal_buf(c).as<double>(); }
+    // the scalar block to the context's pinned copy (synchronizes); a set sweep
+    // error flag is cleared and reported
+    const double* read_scal() {
+        if (!ws) return read_scalars(c);
+        SSGPU_HIP(hipMemcpyAsync(c->h_scal, ws->scal, kScalSlots * sizeof(double), hipMemcpyDeviceToHost, s));
+        SSGPU_HIP(hipStreamSynchronize(s));
+        if (reinterpret_cast<const int*>(c->h_scal)[kErrInt] != 0) {
+            launch_fill<int>(s, 1, reinterpret_cast<int*>(ws->scal) + kErrInt, 0);
+            SSGPU_HIP(hipStreamSynchronize(s));
+            throw Error{HIP_ERROR, "a sync-free sweep exceeded its spin limit (a dependency was never produced)"};
+        }
+        return c->h_scal;
+    }
     // finalize K doubles from nb partials and read them back (synchronizes)
     const double* fetch(int nb, int K) {
-        launch_finalize(s, nb, K, c->partial.as<double>(), scal_buf(c).as<double>());
-        return read_scalars(c);
+        launch_finalize(s, nb, K, partial(), scal());
+        return read_scal();
     }
     template <typename T> double norm(const T* a) {
         launch_norm2<T>(s, m.n, a, partial());
@@ -440,10 +473,11 @@ struct Solver {
     }
     // --- sync-free sweep plumbing ------------------------------------------------
     int* tickets() {
+        if (ws) return ws->tickets;
         if (c->d_tickets.bytes == 0) c->d_tickets.alloc(c, 64);
         return c->d_tickets.as<int>();
     }
-    int* errflag() { return scal_buf(c).as<int>() + kErrInt; }
+    int* errflag() { return reinterpret_cast<int*>(scal()) + kErrInt; }
     bool sf_ok() const { return m.sfL.kmax > 0 && m.sfUic.kmax > 0; }
     template <typename T, typename F>
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
@@ -558,7 +592,7 @@ struct Solver {
         m.prep_kind = 2;
     }
     // throws if a sync-free sweep hit its spin limit (reads back, synchronizes)
-    void check_sweep_error() { (void)read_scalars(c); }
+    void check_sweep_error() { (void)read_scal(); }
 
     // --- shifted IC(0) ------------------------------------------------------
     // Update lists of the factorization (once per matrix); false if they would
@@ -951,6 +985,7 @@ double finish_residual(Solver& S, int solver, const ssgpu_options& o, bool unsca
         o2.diverge_judge_type = 0;
         o2.conv_normalize_type = 0;
         Solver R(m, S.c, o2);
+        R.ws = S.ws;
         T* d = S.vec<T>(13);
         const double cc = std::min(0.5 * conv_cri * nb / rn, 0.5);
         const bool ok = solver == SSGPU_ICCG ? R.iccg_main<T>(cc, max_ite, rn, m.val.as<T>(), rt, d)
@@ -1074,16 +1109,25 @@ int solve_device(Matrix& m, int solver, const ssgpu_options& o, double conv_cri,
 // Solve with the preconditioner built by ssgpu_factorize, on context c's
 // stream and work space.  Nothing of the matrix is written, so solves on
 // different contexts may run at once on one matrix.
+// ws: take every device buffer of the solve from that work space (the
+// allocation-free entry point; timed on the host clock then, no events).
 template <typename T>
 int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite,
-                   double normB, const T* b, T* x, double* log, int log_cap, ssgpu_result* res) {
+                   double normB, const T* b, T* x, double* log, int log_cap, ssgpu_result* res,
+                   Workspace* ws = nullptr) {
     Solver S(m, c, o);
+    S.ws = ws;
     hipStream_t s = c->stream;
     const int n = m.n;
-    hipEvent_t e1, e2;
-    SSGPU_HIP(hipEventCreate(&e1));
-    SSGPU_HIP(hipEventCreate(&e2));
-    SSGPU_HIP(hipEventRecord(e1, s));
+    hipEvent_t e1 = nullptr, e2 = nullptr;
+    const auto h0 = std::chrono::steady_clock::now();
+    if (!ws) {
+        SSGPU_HIP(hipEventCreate(&e1));
+        SSGPU_HIP(hipEventCreate(&e2));
+        SSGPU_HIP(hipEventRecord(e1, s));
+    } else {
+        launch_fill<int>(s, 2 * kScalSlots, reinterpret_cast<int*>(ws->scal), 0);   // the error flag starts clear
+    }
     bool conv = false;
     if (solver == SSGPU_MRTR_SGS) {
         if (m.prep_kind != 2) throw Error{BAD_ARG, "no SGS preparation: call ssgpu_factorize with SSGPU_MRTR_SGS"};
@@ -1108,13 +1152,18 @@ int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, do
     int refinements = 0, refine_its = 0;
     const double true_res = finish_residual<T>(S, solver, o, solver != SSGPU_MRTR_SGS, conv, conv_cri, max_ite,
                                                normB, b, x, refinements, refine_its);
-    SSGPU_HIP(hipEventRecord(e2, s));
-    SSGPU_HIP(hipEventSynchronize(e2));
-    S.check_sweep_error();
     float t_it = 0.f;
-    (void)hipEventElapsedTime(&t_it, e1, e2);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
+    if (!ws) {
+        SSGPU_HIP(hipEventRecord(e2, s));
+        SSGPU_HIP(hipEventSynchronize(e2));
+        S.check_sweep_error();
+        (void)hipEventElapsedTime(&t_it, e1, e2);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+    } else {
+        S.check_sweep_error();
+        t_it = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    }
     if (res) {
         res->converged = conv ? 1 : 0;
         res->status = conv ? OK : NOT_CONVERGED;
@@ -1512,6 +1561,65 @@ struct PermutedIO {
     }
 };
 
+// ------------------------------------------------------------------ scratch
+// Device memory of the allocation-free entry point: one array per GPU, held
+// by the library's code object until the process ends.  Host code never uses
+// its name as an address: the address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept in scratch_dev.
+constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+// this device's scratch address (the device must be current)
+void* scratch_dev(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error{BAD_ARG, "device index beyond the scratch table"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error{HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_dev[device] = p;
+    }
+    return g_scratch_dev[device];
+}
+
+// Lay out a solve's work space in the scratch array: the work vectors the
+// solver, its options and the true-residual pass use, the reduction
+// partials, the scalar block and the tickets.  Throws BAD_ARG when it does
+// not fit.
+void plan_workspace(Workspace& w, char* base, const Matrix& m, int solver, const ssgpu_options& o) {
+    std::vector<int> idx;
+    if (solver == SSGPU_ICCG) idx = {0, 1, 2, 3, 4};
+    else if (solver == SSGPU_MRTR_IC) idx = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    else idx = {0, 1, 2, 3, 4, 5, 6, 10, 14};
+    if (o.is_save_best) idx.push_back(9);
+    if (solver == SSGPU_ICCG && o.is_save_best) idx.push_back(5);
+    idx.push_back(12);   // true residual
+    const bool refinable = o.factor_precision == SSGPU_FACTOR_F32 &&
+                           (solver == SSGPU_ICCG || (solver == SSGPU_MRTR_IC && o.mrtr_direction_fix));
+    if (refinable) idx.push_back(13);
+    for (int k = 0; k < 16; ++k) w.slot_of[k] = -1;
+    int ns = 0;
+    for (int k : idx)
+        if (w.slot_of[k] < 0) w.slot_of[k] = ns++;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t scal_b = up(kScalSlots * sizeof(double)), tick_b = 256;
+    const size_t part_b = up(sizeof(double) * 8 * (size_t)std::max(m.nblk, kMaxReduceBlocks));
+    w.vec_bytes = up(m.vsize * (size_t)std::max(m.n, 1));
+    const size_t need = scal_b + tick_b + part_b + w.vec_bytes * (size_t)ns;
+    if (need > kScratchBytes)
+        throw Error{BAD_ARG, "the allocation-free solve needs " + std::to_string(need) + " bytes of work space for "
+                             "this system (n = " + std::to_string(m.n) + ", " + std::to_string(ns) +
+                             " vectors); the library's scratch holds " + std::to_string(kScratchBytes)};
+    w.scal = reinterpret_cast<double*>(base);
+    w.tickets = reinterpret_cast<int*>(base + scal_b);
+    w.partial = reinterpret_cast<double*>(base + scal_b + tick_b);
+    w.vecs = base + scal_b + tick_b + part_b;
+}
+
 }  // namespace ssgpu
 
 // ==================================================================== C-ABI
@@ -1839,6 +1947,29 @@ int ssgpu_solve_factored(ssgpu_context ctx, ssgpu_matrix mh, int solver, const s
                                             log, log_cap, res);
     if (pio) pio->finish();
     return st;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_solve_factored_noalloc(ssgpu_context ctx, ssgpu_matrix mh, int solver, const ssgpu_options* opt,
+                                 double conv_cri, int max_ite, double normB, const void* d_b, void* d_x, double* log,
+                                 int log_cap, ssgpu_result* res) {
+    if (!ctx || !mh || !d_b || !d_x) return SSGPU_BAD_ARG;
+    const int ck = check_opts(opt, solver);
+    if (ck != SSGPU_OK) return ck;
+    SSGPU_GUARD_BEGIN
+    Matrix& m = mh->m;
+    Context* c = &ctx->c;
+    if (c->device != m.ctx->device) throw Error{BAD_ARG, "context and matrix are on different devices"};
+    if (m.ordered) throw Error{UNSUPPORTED, "the allocation-free solve takes matrices in their own order only"};
+    std::lock_guard<std::mutex> lk(c->mu);
+    SSGPU_HIP(hipSetDevice(c->device));
+    Workspace w;
+    plan_workspace(w, static_cast<char*>(scratch_dev(c->device)), m, solver, *opt);
+    return m.dtype == SSGPU_R64
+               ? solve_factored<double>(c, m, solver, *opt, conv_cri, max_ite, normB, (const double*)d_b,
+                                        (double*)d_x, log, log_cap, res, &w)
+               : solve_factored<zc>(c, m, solver, *opt, conv_cri, max_ite, normB, (const zc*)d_b, (zc*)d_x, log,
+                                    log_cap, res, &w);
     SSGPU_GUARD_END
 }
 
