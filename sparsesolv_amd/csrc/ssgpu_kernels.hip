@@ -114,17 +114,19 @@ __device__ inline void spmv_block(int b, const int* __restrict__ blk, const int*
         for (int base = 0; base < ne; base += kStage) {
             int cc[U];
             T vv[U], xx[U];
+            // branch-free, so that all U loads (then all U gathers) are in
+            // flight at once: a slot past the block's entries re-reads its
+            // last entry (ne >= 1 here) and its product is not stored
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = base + u * kBS + (int)threadIdx.x;
-                if (q < ne) {   // the matrix streams through once: keep it out of the caches' way
-                    cc[u] = __builtin_nontemporal_load(ci + e0 + q);
-                    vv[u] = ntload(v + e0 + q);
-                }
+                const int qc = q < ne ? q : ne - 1;
+                // the matrix streams through once: keep it out of the caches' way
+                cc[u] = __builtin_nontemporal_load(ci + e0 + qc);
+                vv[u] = ntload(v + e0 + qc);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (base + u * kBS + (int)threadIdx.x < ne) xx[u] = x[cc[u]];
+            for (int u = 0; u < U; ++u) xx[u] = x[cc[u]];
             if (base > 0) __syncthreads();   // the previous piece has been summed
 #pragma unroll
             for (int u = 0; u < U; ++u) {

@@ -69,7 +69,7 @@ template <> __device__ inline zc wid<zc, fc>(fc a) { return zc{(double)a.re, (do
 // Row state: the lane's row of the chunk, its ELL entries in registers.
 template <typename T, typename F, int MODE, int KMAX> struct RowState {
     int i = 0, len = 0;
-    int cols[KMAX];
+    unsigned cols[KMAX];   // unsigned: widening to an address offset needs no instruction on the loaded value
     F vals[KMAX];
     T r_i, a_i, d_i;
     F dg;
@@ -82,13 +82,19 @@ template <typename T, typename F, int MODE, int KMAX> struct RowState {
         dg = F();
         i = active ? srow[p] : 0;
         len = active ? rlen[p] : 0;
-        const int base = active ? slice_off[c] + lane : 0;   // inactive lanes read nothing
+        const int base = active ? slice_off[c] + lane : 0;
+        // Every lane issues all KMAX loads back to back, with no branch and
+        // no wait between them: slots past the row's length (and inactive
+        // lanes) read entry 0, which exists in every sliced-ELL copy (its
+        // arrays hold at least one element).  A per-slot branch here let the
+        // compiler finish each load before issuing the next: KMAX serial
+        // memory round trips per chunk.
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < len) {
-                cols[k] = ecol[base + kSfRows * k];
-                vals[k] = eval[base + kSfRows * k];
-            }
+        for (int k = 0; k < KMAX; ++k) {
+            const int at = k < len ? base + kSfRows * k : 0;
+            cols[k] = (unsigned)ecol[at];
+            vals[k] = eval[at];
+        }
         if (active) {
             if (MODE == BWD_IC) {
                 a_i = aux[i];
@@ -258,10 +264,11 @@ __global__ __launch_bounds__(kSfRows * WPB) void k_sweep_lvl(int p0, int p1, con
     RowState<T, F, MODE, KMAX> row;
     row.load(p, active, c, lane, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd);
     if (!active) return;
+    // all KMAX gathers at once: a padding slot holds a valid column (that of
+    // entry 0), and result() ignores slots past the row's length
     T g[KMAX];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k)
-        if (k < row.len) g[k] = x[row.cols[k]];
+    for (int k = 0; k < KMAX; ++k) g[k] = x[row.cols[k]];
     x[row.i] = row.result(g);
 }
 
@@ -290,15 +297,15 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int
         const bool active = p < n;
         const int i = active ? srow[p] : 0;
         const int len = active ? rlen[p] : 0;
-        int cols[KMAX];
+        unsigned cols[KMAX];
         F vals[KMAX];
-        const int base = active ? slice_off[c] + lane : 0;   // inactive lanes read nothing
+        const int base = active ? slice_off[c] + lane : 0;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < len) {
-                cols[k] = ecol[base + kSfRows * k];
-                vals[k] = eval[base + kSfRows * k];
-            }
+        for (int k = 0; k < KMAX; ++k) {   // branch-free, as in RowState::load
+            const int at = k < len ? base + kSfRows * k : 0;
+            cols[k] = (unsigned)ecol[at];
+            vals[k] = eval[at];
+        }
         T rv[KC];
         T d_i = Num<T>::zero();
         F dg = F();
