@@ -260,20 +260,29 @@ def _engine_run(eng, reqs, arrivals=None):
 
 def _build_parallel(specs, precision, threads=16):
     """Upload, analyse and factorize many systems at once: one library
-    context per thread, so the builds overlap on the host and on the GPU.
+    context per worker thread, so the builds overlap on the host and on the
+    GPU and each thread's tuners time its own stream only.
     specs: [(system, shift)]; returns (matrices, contexts, seconds)."""
+    import threading
     from concurrent.futures import ThreadPoolExecutor
     from sparsesolv_amd import capi
-    ctxs = [capi.Context(0) for _ in range(min(threads, len(specs)))]
+    ctxs, local, lock = [], threading.local(), threading.Lock()
     mats = [None] * len(specs)
+
+    def context():
+        if not hasattr(local, "ctx"):
+            local.ctx = capi.Context(0)
+            with lock:
+                ctxs.append(local.ctx)
+        return local.ctx
 
     def build(i):
         s, acc = specs[i]
-        m = capi.Matrix(ctxs[i % len(ctxs)], s.row_ptr, s.col_idx, s.values)
+        m = capi.Matrix(context(), s.row_ptr, s.col_idx, s.values)
         m.factorize(capi.ICCG, acc, precision)
         mats[i] = m
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(len(ctxs)) as ex:
+    with ThreadPoolExecutor(max(1, min(threads, len(specs)))) as ex:
         list(ex.map(build, range(len(specs))))
     for c in ctxs:
         c.synchronize()
@@ -305,7 +314,7 @@ def batched_run(a, nsys=256, max_ite=2000):
     sizes = [s.n for s, _ in specs]
     print(f"# batched: {nsys} systems N {min(sizes)}..{max(sizes)} (sum {sum(sizes)}), plus N={big.n} x 32 rhs "
           f"(generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
-    mats, ctxs, t_build = _build_parallel(specs + [(big, 1.15)], capi.FACTOR_F32)
+    mats, ctxs, t_build = _build_parallel(specs + [(big, 1.15)], capi.FACTOR_F32, threads=a.build_threads)
     print(f"# batched: {len(mats)} matrices uploaded, analysed and factorized in {t_build:.1f}s", file=sys.stderr,
           flush=True)
     rhs = [s.rhs for s, _ in specs] + big_rhs
@@ -634,6 +643,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--nmax", type=int, default=120, help="serving: largest grid edge")
+    ap.add_argument("--build-threads", type=int, default=16,
+                    help="batched: host threads building the systems (each with its own context)")
     ap.add_argument("--batched", type=int, default=1,
                     help="default line: also run the batched configuration (256 systems + 60^3 x 32 rhs)")
     ap.add_argument("--poisson", type=int, default=64,

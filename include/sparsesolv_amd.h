@@ -96,14 +96,24 @@ const char* ssgpu_last_error(void);
 int ssgpu_context_create(int device, ssgpu_context* out);
 int ssgpu_context_destroy(ssgpu_context ctx);
 int ssgpu_context_synchronize(ssgpu_context ctx);
-/* Bytes of device memory currently held by the context's matrices. */
+/* Bytes of device memory currently held by the context: the matrices
+ * created on it (with their preconditioners) and its work pool. */
 int ssgpu_context_resident_bytes(ssgpu_context ctx, int64_t* bytes);
+/* Free the context's work pool (Krylov vectors, reduction partials and block
+ * scratch kept between solves); the next solve grows it again.  Waits for a
+ * call running on the context to return.  Matrices are not touched.  (No
+ * counterpart in the reference, whose work vectors live per solve call:
+ * MatSolverICCG.cpp:191-200.  For callers that cap device memory.) */
+int ssgpu_context_trim(ssgpu_context ctx);
+/* Device memory held by the library in this process, all contexts: bytes
+ * now, and the high-water mark since the process started or since the last
+ * call with reset_peak != 0 (which then restarts it from the bytes now). */
+int ssgpu_memory(int64_t* resident, int64_t* peak, int reset_peak);
 
 /* Upload a square CSR matrix and analyse its structure (lower/upper split,
  * transpose map, level schedules).  Values are copied to the device. */
 int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr,
                         const int32_t* col_idx, const void* values, ssgpu_matrix* out);
-/* Replace the values (same pattern) from host memory. */
 /* Same matrix in the algebraic block multi-color (ABMC) order: rows
  * aggregated into blocks of at most block_size, blocks colored so that a
  * color's blocks share no entry (Iwashita et al., IPDPS 2012; the reference
@@ -123,6 +133,8 @@ int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const i
                                 const int32_t* col_idx, const void* values, int block_size,
                                 ssgpu_matrix* out, int32_t* ncolors);
 
+/* Replace the values (same pattern) from host memory.  A factor built from
+ * the old values is dropped: cached solves need ssgpu_factorize again. */
 int ssgpu_matrix_update_values(ssgpu_matrix m, const void* values);
 int ssgpu_matrix_destroy(ssgpu_matrix m);
 int ssgpu_matrix_info(ssgpu_matrix m, int32_t* n, int64_t* nnz, int* dtype,
@@ -239,6 +251,10 @@ int ssgpu_batch_poll(ssgpu_batch b, int* got, int64_t* tag, ssgpu_result* res);
  * capture the runtime refused (those steps ran the same kernels eagerly). */
 int ssgpu_batch_info(ssgpu_batch b, int* systems, int64_t* iterations_run, int* graph_fallbacks);
 int ssgpu_batch_destroy(ssgpu_batch b);
+/* Free the device vectors of the session's idle slots (kept for reuse by the
+ * next system added); systems in the session are not touched.  For callers
+ * that cap device memory between bursts of work. */
+int ssgpu_batch_trim(ssgpu_batch b);
 /* One-shot form: nsys systems with shared options, tolerance and iteration
  * limit; res[i] for system i.  Returns SSGPU_OK when every system converged,
  * SSGPU_NOT_CONVERGED otherwise. */

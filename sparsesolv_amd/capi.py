@@ -68,6 +68,8 @@ def lib():
         L.ssgpu_context_destroy.argtypes = [V]
         L.ssgpu_context_synchronize.argtypes = [V]
         L.ssgpu_context_resident_bytes.argtypes = [V, P(ctypes.c_int64)]
+        L.ssgpu_context_trim.argtypes = [V]
+        L.ssgpu_memory.argtypes = [P(ctypes.c_int64), P(ctypes.c_int64), I]
         L.ssgpu_matrix_create.argtypes = [V, I, ctypes.c_int32, V, V, V, P(V)]
         L.ssgpu_matrix_create_ordered.argtypes = [V, I, ctypes.c_int32, V, V, V, I, P(V), P(ctypes.c_int32)]
         L.ssgpu_abmc_order.argtypes = [ctypes.c_int32, V, V, I, V, P(ctypes.c_int32)]
@@ -87,6 +89,7 @@ def lib():
         L.ssgpu_batch_step.argtypes = [V, I, P(I), P(I)]
         L.ssgpu_batch_poll.argtypes = [V, P(I), P(ctypes.c_int64), P(Result)]
         L.ssgpu_batch_info.argtypes = [V, P(I), P(ctypes.c_int64), P(I)]
+        L.ssgpu_batch_trim.argtypes = [V]
         L.ssgpu_batch_destroy.argtypes = [V]
         L.ssgpu_solve_batch.argtypes = [V, I, I, P(V), P(Options), D, I, V, P(V), P(V), P(Result)]
         L.ssgpu_spmv_device.argtypes = [V, V, V]
@@ -142,6 +145,18 @@ class Context:
         b = ctypes.c_int64()
         check(lib().ssgpu_context_resident_bytes(self.h, ctypes.byref(b)))
         return b.value
+
+    def trim(self):
+        """Free the work pool kept between solves (matrices stay)."""
+        check(lib().ssgpu_context_trim(self.h))
+
+
+def memory(reset_peak: bool = False):
+    """(bytes now, high-water mark) of device memory held by the library in
+    this process; reset_peak restarts the mark from the bytes now."""
+    now, peak = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().ssgpu_memory(ctypes.byref(now), ctypes.byref(peak), int(bool(reset_peak))))
+    return now.value, peak.value
 
 
 def _dtype_code(values):
@@ -345,6 +360,10 @@ class Batch:
         n, its, fb = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
         check(lib().ssgpu_batch_info(self.h, ctypes.byref(n), ctypes.byref(its), ctypes.byref(fb)))
         return n.value, its.value, fb.value
+
+    def trim(self):
+        """Free the device vectors of slots no system occupies."""
+        check(lib().ssgpu_batch_trim(self.h))
 
     def close(self):
         if self.h:

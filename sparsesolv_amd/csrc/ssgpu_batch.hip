@@ -668,6 +668,25 @@ int ssgpu_batch_info(ssgpu_batch bh, int* systems, int64_t* iterations_run, int*
     return SSGPU_OK;
 }
 
+int ssgpu_batch_trim(ssgpu_batch bh) {
+    if (!bh) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    Batch& B = bh->b;
+    std::lock_guard<std::mutex> lk(B.c->mu);
+    SSGPU_HIP(hipSetDevice(B.c->device));
+    SSGPU_HIP(hipStreamSynchronize(B.c->stream));   // no queued launch may still use a slot's vectors
+    bool freed = false;
+    for (auto& q : B.slots) {
+        if (q.state != SLOT_FREE) continue;
+        freed |= (q.vecs && q.vecs->bytes) || (q.part && q.part->bytes);
+        q.vecs.reset();
+        q.part.reset();
+    }
+    if (freed) B.dirty = true;   // the device tables and the graph are rebuilt before the next step
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
 int ssgpu_batch_destroy(ssgpu_batch bh) {
     if (!bh) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN

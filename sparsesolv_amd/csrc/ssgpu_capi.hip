@@ -32,6 +32,9 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string& s) { g_last_error = s; }
 
 // ------------------------------------------------------------------ memory
+// device bytes held by the library in this process, and their high-water mark
+static std::atomic<int64_t> g_resident{0}, g_peak{0};
+
 void DevBuf::alloc(Context* c, size_t b) {
     release();
     ctx = c;
@@ -40,11 +43,16 @@ void DevBuf::alloc(Context* c, size_t b) {
     SSGPU_HIP(hipMalloc(&p, b));
     bytes = b;
     c->resident += (int64_t)b;
+    const int64_t now = g_resident += (int64_t)b;
+    int64_t pk = g_peak.load();
+    while (now > pk && !g_peak.compare_exchange_weak(pk, now)) {
+    }
 }
 void DevBuf::release() {
     if (p) {
         (void)hipFree(p);
         if (ctx) ctx->resident -= (int64_t)bytes;
+        g_resident -= (int64_t)bytes;
     }
     p = nullptr;
     bytes = 0;
@@ -1719,6 +1727,29 @@ int ssgpu_context_resident_bytes(ssgpu_context ctx, int64_t* bytes) {
     return SSGPU_OK;
 }
 
+int ssgpu_context_trim(ssgpu_context ctx) {
+    if (!ctx) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    Context& c = ctx->c;
+    std::lock_guard<std::mutex> lk(c.mu);
+    SSGPU_HIP(hipSetDevice(c.device));
+    SSGPU_HIP(hipStreamSynchronize(c.stream));   // nothing queued may still use the pool
+    c.work.clear();
+    c.partial.release();
+    c.d_bpartial.release();
+    c.d_btickets.release();
+    c.d_bscal.release();
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_memory(int64_t* resident, int64_t* peak, int reset_peak) {
+    if (resident) *resident = g_resident.load();
+    if (peak) *peak = g_peak.load();
+    if (reset_peak) g_peak.store(g_resident.load());
+    return SSGPU_OK;
+}
+
 int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                         const void* values, ssgpu_matrix* out) {
     if (!ctx || !out || !row_ptr || n <= 0 || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
@@ -1742,6 +1773,8 @@ int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* 
             if (k > row_ptr[i] && col_idx[k] <= col_idx[k - 1])
                 throw Error{BAD_ARG, "column indices must be strictly ascending within a row"};
         }
+    // the context's stream carries the upload and the SpMV tuner: one call at a time
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
     SSGPU_HIP(hipSetDevice(ctx->c.device));
     upload(m.ctx, m.rp, m.h_rp);
     upload(m.ctx, m.ci, m.h_ci);

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -33,7 +34,7 @@ struct DevBuf {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
-    int64_t resident = 0;
+    std::atomic<int64_t> resident{0};   // bytes of the DevBufs allocated on it (any thread may free one)
     double* h_scal = nullptr;   // pinned scalar read-back (kScalSlots doubles)
     DevBuf d_scal;              // device scalar results; the last slot holds the sweep error flag
     DevBuf d_count;

@@ -26,10 +26,12 @@ Each result is exactly what a solo solve against the same factor returns
 a request's work never touches another's vectors; the factor is only read).
 
 Memory: device bytes are counted on every context of the engine (matrices,
-preconditioners, work vectors, batch tables and vectors), and a system is
-admitted only while that count plus an estimate of the new system fits the
-HBM budget; least recently used systems nobody is solving on are evicted to
-make room, and a request waits while the budget is held by systems in use.
+preconditioners, work pools, batch tables and vectors), and every allocation
+a request causes (its system with the preconditioner, a solo solve's work
+pool, a batch slot) is reserved from the HBM budget beforehand, with upper
+bounds measured for this library.  What does not fit evicts least recently
+used systems nobody is solving on, then frees idle work pools and batch
+slots, then waits for memory in use (see Engine).
 
 The reference has no serving layer; its solvers refactor on every call
 (MatSolverICCG.cpp:109-118).
@@ -73,18 +75,29 @@ class Response:
     error: Optional[str] = None
 
 
-def _system_bytes(nnz: int, n: int, cplx: bool) -> int:
-    """Device bytes of a resident system with its preconditioner: CSR, the
-    level-ordered lower / transposed copies, two sliced-ELL copies, the
-    factor and its update lists (measured on the curl-curl systems: about
-    60 B per stored entry for real values, 95 B for complex)."""
-    return int((95 if cplx else 60) * nnz + 64 * n)
+def _system_bytes(nnz: int, n: int, cplx: bool, precision: int = capi.FACTOR_F64) -> int:
+    """Upper bound of the device bytes of a resident system with its
+    preconditioner: CSR, the level-ordered lower / transposed copies, two
+    sliced-ELL copies, the factor and its update lists (an FP32 factor keeps
+    the FP64 copy too).  Measured on curl-curl and eddy-current systems of
+    240 to 626,580 unknowns, for every solver: at most 0.96 of this bound."""
+    f32 = precision == capi.FACTOR_F32
+    per_entry = (130 if f32 else 114) if cplx else (94 if f32 else 84)
+    return int(per_entry * nnz + 64 * n + (192 << 10))
+
+
+def _pool_bytes(n: int, cplx: bool) -> int:
+    """Upper bound of a context's work pool after solving a system of n
+    unknowns: at most 14 vectors (the MRTR solvers' 12 with the best-iterate
+    copy and refinement, plus the two permuted copies of an ABMC-ordered
+    matrix) and the reduction partials."""
+    return 14 * (16 if cplx else 8) * n + (128 << 10)
 
 
 def _batch_vector_bytes(n: int, cplx: bool) -> int:
-    """Device bytes a system holds while it is in a batch: eight vectors of n
+    """Device bytes a system's slot holds in a batch: eight vectors of n
     values (b, x, p, r, z, Ap, sweep scratch, best iterate) and its partials."""
-    return (16 if cplx else 8) * n * 8 + 64 * (n // 128 + 1024)
+    return (16 if cplx else 8) * n * 8 + 64 * (n // 128 + 1024) + 4096
 
 
 def rhs_error(rhs, matrix) -> Optional[str]:
@@ -123,6 +136,10 @@ class _Runner:
         self.inbox: "collections.deque" = collections.deque()
         self.closing = False
         self.ctx = None
+        self.batch = None
+        self.hlock = threading.Lock()   # guards the ctx / batch handles against trims from other threads
+        self.pool_hold = 0           # bytes reserved for the context's work pool (results' true residuals)
+        self._resident = 0           # last device bytes read from the context
         self.occupancy = []          # systems in the batch at each check
         self.failures = []           # runner failures outside the solves (each restarted the session)
         self.graph_fallbacks = 0     # graph captures that fell back to eager launches
@@ -141,19 +158,40 @@ class _Runner:
         self.thread.join()
 
     def resident(self) -> int:
-        c = self.ctx
-        return c.resident_bytes() if c is not None else 0
+        # never waits: a trim holds the lock while the session's stream drains
+        if self.hlock.acquire(blocking=False):
+            try:
+                self._resident = self.ctx.resident_bytes() if self.ctx is not None else 0
+            finally:
+                self.hlock.release()
+        return self._resident
+
+    def trim(self):
+        """Free the idle slots' vectors and the context's work pool."""
+        with self.hlock:
+            if self.batch is not None:
+                self.batch.trim()
+            if self.ctx is not None:
+                self.ctx.trim()
+
+    def _new_batch(self):
+        with self.hlock:
+            if self.batch is not None:
+                self.graph_fallbacks += self.batch.info()[2]
+                self.batch.close()
+            self.batch = capi.Batch(self.ctx, self.complex, self.precision)
 
     def _run(self):
         eng = self.eng
-        self.ctx = capi.Context(eng.device)
-        batch = capi.Batch(self.ctx, self.complex, self.precision)
+        with self.hlock:
+            self.ctx = capi.Context(eng.device)
+        self._new_batch()
         inflight = {}
         tag = 0
         try:
             while True:
                 try:
-                    tag, stop = self._turn(batch, inflight, tag)
+                    tag, stop = self._turn(inflight, tag)
                 except Exception as ex:   # noqa: BLE001 - every waiting request hears of it
                     # a failure outside the solves themselves: answer every request of
                     # the batch and the inbox with it, and start a fresh session
@@ -168,18 +206,27 @@ class _Runner:
                         eng._respond(fut, r, None, None, hit, f"batch runner failed: {ex}")
                         eng._release(key, e)
                     inflight.clear()
-                    batch.close()
-                    batch = capi.Batch(self.ctx, self.complex, self.precision)
+                    self._set_pool_hold(0)
+                    self._new_batch()
                     stop = False
                 if stop:
                     break
         finally:
-            self.graph_fallbacks += batch.info()[2]
-            batch.close()
-            self.ctx.close()
-            self.ctx = None
+            self._set_pool_hold(0)
+            with self.hlock:
+                self.graph_fallbacks += self.batch.info()[2]
+                self.batch.close()
+                self.batch = None
+                self.ctx.close()
+                self.ctx = None
 
-    def _turn(self, batch, inflight, tag):
+    def _set_pool_hold(self, nbytes):
+        if nbytes != self.pool_hold:
+            self.eng._unreserve(self.pool_hold)
+            self.eng._reserve(nbytes, force=True)
+            self.pool_hold = nbytes
+
+    def _turn(self, inflight, tag):
         """Admit what the inbox holds, run one step, answer the finished; (next tag, stop)."""
         eng = self.eng
         with self.cv:
@@ -189,14 +236,22 @@ class _Runner:
                 return tag, True
             items = list(self.inbox)
             self.inbox.clear()
+        batch = self.batch
         keep = []
         for item in items:
             r, fut, key, e, hit = item
             n = e.matrix.n
-            # a new system's vectors must fit the budget too, unless the batch is empty
-            if inflight and not eng._room(_batch_vector_bytes(n, self.complex)):
+            # the slot's vectors, and the work pool's growth for this system's
+            # result, are reserved from the budget before they are allocated;
+            # a request that does not fit waits for the batch to shrink (an
+            # empty batch takes it regardless, or nothing would progress)
+            vec = _batch_vector_bytes(n, self.complex)
+            pool = max(self.pool_hold, _pool_bytes(n, self.complex))
+            extra = vec + pool - self.pool_hold
+            if not eng._reserve(extra, force=not inflight and not keep):
                 keep.append(item)
                 continue
+            self.pool_hold = pool   # its growth is part of what was just reserved
             b = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
             x = np.zeros(n, e.matrix.dtype)
             opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
@@ -207,9 +262,12 @@ class _Runner:
                 eng._respond(fut, r, None, None, hit, str(ex))
                 eng._release(key, e)
                 continue
+            finally:
+                eng._unreserve(vec)   # allocated by now (or not at all)
             inflight[tag] = (r, fut, key, e, hit, b, x)
             tag += 1
         if keep:
+            eng._trim_idle(caller=self)
             with self.cv:
                 self.inbox.extendleft(reversed(keep))
         if not inflight:
@@ -218,7 +276,6 @@ class _Runner:
         try:
             running, _ = batch.step(eng.check_every)
             self.occupancy.append(running)
-            eng._note_resident()
         except capi.SolverError as ex:
             err = str(ex)
         for t, res in batch.poll():
@@ -228,10 +285,29 @@ class _Runner:
             else:
                 eng._respond(fut, r, None, res, hit, err or f"batched solve failed (status {res.status})")
             eng._release(key, e)
+        if not inflight:
+            # the batch drained: its pool needs no reservation any more, and
+            # under memory pressure its vectors and pool go back too
+            self._set_pool_hold(0)
+            if eng._pressed():
+                self.trim()
         return tag, False
 
 
 class Engine:
+    """Serving engine on one GPU (see the module docstring).
+
+    Device memory: every allocation the engine causes is reserved from
+    ``hbm_budget`` before it is made, with upper bounds measured for this
+    library (a system with its preconditioner, a context's work pool, a batch
+    slot), and counted from the library's contexts once made.  A system that
+    does not fit evicts idle systems (least recently used first), then frees
+    the idle work pools and batch slots, then waits for systems in use; when
+    nothing else holds memory or reservations it is admitted regardless,
+    which ``over_budget`` counts.  ``peak_resident`` is the high-water mark of
+    the library's device memory since the engine started, above what was
+    held when it started."""
+
     def __init__(self, device: int = 0, slots: int = 8, hbm_budget: Optional[int] = None, batching: bool = True,
                  check_every: int = 4):
         import torch
@@ -243,14 +319,17 @@ class Engine:
         self.check_every = check_every
         self.lock = threading.Condition()
         self.cache: "collections.OrderedDict[object, _Entry]" = collections.OrderedDict()
-        self.building = 0            # estimated bytes of systems being built (not yet counted on a context)
-        self.peak_resident = 0
+        self.reserved = 0            # bytes reserved for allocations not made yet
+        self.over_budget = 0         # admissions made although the budget was full
+        self._mem0, _ = capi.memory(reset_peak=True)
         self.pending: "collections.deque" = collections.deque()
         self.qcv = threading.Condition()
         self.closing = False
         self.solo_solves = 0
         self.runners = {}
-        self.contexts = []
+        self.contexts = {}           # builder context -> lock guarding its handle
+        self._ctx_bytes = {}         # builder context -> last device bytes read from it
+        self.idle_ctx = set()        # builder contexts waiting for a request
         self.workers = []
         for _ in range(slots):
             t = threading.Thread(target=self._worker, daemon=True)
@@ -296,6 +375,12 @@ class Engine:
         """Graph captures of closed runners that fell back to eager launches."""
         return sum(r.graph_fallbacks for r in list(self.runners.values()))
 
+    @property
+    def peak_resident(self) -> int:
+        """High-water mark of the library's device bytes since the engine
+        started, above those held when it started."""
+        return capi.memory()[1] - self._mem0
+
     def close(self):
         with self.qcv:
             self.closing = True
@@ -312,21 +397,62 @@ class Engine:
 
     # ------------------------------------------------------------ internals
     def resident(self) -> int:
-        """Device bytes held by the engine's contexts right now."""
+        """Device bytes held by the engine's contexts right now (a context
+        being trimmed counts with its bytes before the trim)."""
         total = 0
-        for c in list(self.contexts):
-            total += c.resident_bytes()
+        for c, lk in list(self.contexts.items()):
+            if lk.acquire(blocking=False):   # never waits for a trim
+                try:
+                    if c.h:
+                        self._ctx_bytes[c] = c.resident_bytes()
+                finally:
+                    lk.release()
+            total += self._ctx_bytes.get(c, 0)
         for r in list(self.runners.values()):
             total += r.resident()
         return total
 
-    def _note_resident(self):
-        now = self.resident() + self.building
-        if now > self.peak_resident:
-            self.peak_resident = now
+    def _pressed(self) -> bool:
+        """Memory pressure: more than 3/4 of the budget held or reserved."""
+        return 4 * (self.resident() + self.reserved) > 3 * self.budget
 
     def _room(self, extra: int) -> bool:
-        return self.resident() + self.building + extra <= self.budget
+        return self.resident() + self.reserved + extra <= self.budget
+
+    def _reserve(self, nbytes: int, force: bool = False) -> bool:
+        """Reserve nbytes of the budget for an allocation about to be made;
+        force reserves even past the budget (counted in over_budget)."""
+        with self.lock:
+            if nbytes <= 0:
+                return True
+            if self._room(nbytes):
+                self.reserved += nbytes
+                return True
+            if force:
+                self.reserved += nbytes
+                self.over_budget += 1
+                return True
+            return False
+
+    def _unreserve(self, nbytes: int):
+        if nbytes > 0:
+            with self.lock:
+                self.reserved -= nbytes
+                self.lock.notify_all()
+
+    def _trim_idle(self, own_ctx=None, caller=None):
+        """Free the work pools of idle builders (waiting for requests) and of
+        the caller's own context, and the idle slots of the batch runners;
+        never waits for a builder's solve."""
+        for c, lk in list(self.contexts.items()):
+            if c is not own_ctx and c not in self.idle_ctx:
+                continue
+            with lk:
+                if c.h:
+                    c.trim()
+        for r in list(self.runners.values()):
+            if r is not caller:
+                r.trim()
 
     def _respond(self, fut, r: Request, x, res, hit, err):
         fut.set_result(Response(x, res, time.perf_counter() - r.arrival, hit, err))
@@ -350,39 +476,72 @@ class Engine:
                 return None
             return self.pending.popleft()
 
-    def _acquire(self, r: Request, ctx: capi.Context):
-        """The resident system for r (built on the caller's context and stream
-        if needed, so misses in different builders build in parallel), with a
-        user reference."""
-        key = self._cache_key(r)
-        cplx = np.iscomplexobj(r.values)
-        need = _system_bytes(len(r.values), len(r.row_ptr) - 1, cplx)
-        if need > self.budget:
-            raise MemoryError(f"system needs {need} B, budget {self.budget} B")
+    def _reserve_wait(self, nbytes: int, holding: bool, own_ctx=None, patience: float = 2.0):
+        """Reserve nbytes, evicting idle systems and trimming idle pools as
+        needed, else waiting for memory in use; admitted regardless when
+        nothing else holds reservations or systems (`holding`: the caller
+        holds a system itself), or after `patience` seconds of waiting (two
+        callers that each hold a system could otherwise wait for each other)."""
+        trimmed = False
+        deadline = time.perf_counter() + patience
         with self.lock:
             while True:
-                e = self.cache.get(key)
-                if e is not None:
-                    e.users += 1
-                    self.cache.move_to_end(key)
-                    hit = True
-                    break
-                # evict idle systems, least recently used first, until it fits
-                while not self._room(need):
+                while not self._room(nbytes):
                     idle = [k for k, v in self.cache.items() if v.users == 0 and v.ready.is_set()]
                     if not idle:
                         break
                     v = self.cache.pop(idle[0])
                     if v.matrix is not None:
                         v.matrix.close()
-                if self._room(need) or not any(v.users for v in self.cache.values()):
+                if self._room(nbytes):
+                    self.reserved += nbytes
+                    return
+                if not trimmed:
+                    trimmed = True
+                    self.lock.release()
+                    try:
+                        self._trim_idle(own_ctx)
+                    finally:
+                        self.lock.acquire()
+                    continue
+                others = sum(v.users for v in self.cache.values()) - (1 if holding else 0)
+                if (others <= 0 and self.reserved == 0) or time.perf_counter() > deadline:
+                    self.reserved += nbytes
+                    self.over_budget += 1
+                    return
+                self.lock.wait(0.05)   # budget held by systems in use
+
+    def _acquire(self, r: Request, ctx: capi.Context):
+        """The resident system for r (built on the caller's context and stream
+        if needed, so misses in different builders build in parallel), with a
+        user reference."""
+        key = self._cache_key(r)
+        cplx = np.iscomplexobj(r.values)
+        need = _system_bytes(len(r.values), len(r.row_ptr) - 1, cplx, r.precision)
+        if need > self.budget:
+            raise MemoryError(f"system needs {need} B, budget {self.budget} B")
+        with self.lock:
+            e = self.cache.get(key)
+            if e is not None:
+                e.users += 1
+                self.cache.move_to_end(key)
+        if e is None:
+            self._reserve_wait(need, holding=False, own_ctx=ctx)
+            with self.lock:
+                e = self.cache.get(key)
+                if e is not None:   # built by another builder meanwhile
+                    e.users += 1
+                    self.cache.move_to_end(key)
+                    self.reserved -= need
+                    self.lock.notify_all()
+                    hit = True
+                else:
                     e = _Entry(None, need)
                     e.users = 1
                     self.cache[key] = e
-                    self.building += need
                     hit = False
-                    break
-                self.lock.wait(0.05)   # budget held by systems in use
+        else:
+            hit = True
         if not hit:
             try:
                 m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values, abmc_block=r.abmc_block)
@@ -390,9 +549,7 @@ class Engine:
                 e.matrix = m
             except Exception as ex:   # noqa: BLE001 - reported to every waiter
                 e.error = str(ex)
-            with self.lock:
-                self.building -= need
-                self._note_resident()
+            self._unreserve(need)
             e.ready.set()
         else:
             e.ready.wait()
@@ -413,11 +570,14 @@ class Engine:
     def _worker(self):
         torch = self.torch
         ctx = capi.Context(self.device)
+        lk = threading.Lock()
         with self.lock:
-            self.contexts.append(ctx)
+            self.contexts[ctx] = lk
         stream = torch.cuda.Stream(device=self.device)
         while True:
+            self.idle_ctx.add(ctx)
             item = self._take()
+            self.idle_ctx.discard(ctx)
             if item is None:
                 break
             r, fut = item
@@ -436,6 +596,8 @@ class Engine:
             if self.batching and r.solver == capi.ICCG and not r.opts.get("is_diag_scale", 0):
                 self._runner(e.matrix.dtype == np.complex128, r.precision).put((r, fut, key, e, hit))
                 continue
+            pool = _pool_bytes(e.matrix.n, e.matrix.dtype == np.complex128)
+            self._reserve_wait(pool, holding=True, own_ctx=ctx)
             try:
                 opts = capi.default_options(accel_factor=r.accel, factor_precision=r.precision, **r.opts)
                 rhs = np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype)
@@ -454,10 +616,13 @@ class Engine:
             except Exception as ex:  # noqa: BLE001
                 self._respond(fut, r, None, None, hit, str(ex))
             finally:
+                self._unreserve(pool)
                 self._release(key, e)
         with self.lock:
-            self.contexts.remove(ctx)
-        ctx.close()
+            self.contexts.pop(ctx, None)
+            self._ctx_bytes.pop(ctx, None)
+        with lk:
+            ctx.close()
 
 
 def percentiles(lat):

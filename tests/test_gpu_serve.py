@@ -118,8 +118,9 @@ def test_serving_mix_matches_solo(torch_cuda, ctx):
     eng.close()
     hits = sum(r.cache_hit for r in resp)
     print(f"serving mix: {len(resp)} requests, {hits} cache hits, peak resident {eng.peak_resident} B of "
-          f"{eng.budget} B, errors {[r.error for r in resp if r.error]}, runner failures {eng.runner_failures}, "
-          f"graph fallbacks {eng.graph_fallbacks}", flush=True)
+          f"{eng.budget} B, over-budget admissions {eng.over_budget}, errors {[r.error for r in resp if r.error]}, "
+          f"runner failures {eng.runner_failures}, graph fallbacks {eng.graph_fallbacks}", flush=True)
+    assert eng.peak_resident <= eng.budget
     for (key, solver, prec, b), r in zip(plan, resp):
         assert r.error is None, r.error
         x_solo, r_solo = _solo(torch_cuda, ctx, systems[key], b, solver, 1.1, prec)
@@ -280,9 +281,10 @@ def test_serving_poisson_stream_against_oracle(torch_cuda, ctx):
     lat = serve.percentiles([r.latency_s for r in resp])
     print(f"poisson stream: {len(resp)} requests, {len(cache)} systems, {sum(r.cache_hit for r in resp)} hits, "
           f"max batch {max(occupancy)}, p50 {1e3 * lat['p50']:.0f} ms p99 {1e3 * lat['p99']:.0f} ms, peak resident "
-          f"{eng.peak_resident} B of {eng.budget} B, runner failures {eng.runner_failures}, graph fallbacks "
-          f"{eng.graph_fallbacks}", flush=True)
+          f"{eng.peak_resident} B of {eng.budget} B, over-budget admissions {eng.over_budget}, runner failures "
+          f"{eng.runner_failures}, graph fallbacks {eng.graph_fallbacks}", flush=True)
     assert max(occupancy) > 1
+    assert eng.peak_resident <= eng.budget
     for (key, b, prec), r in zip(plan, resp):
         assert r.error is None, r.error
         s = cache[key]
