@@ -176,6 +176,7 @@ def served_run(a, s, code, release):
            for k in range(a.serve_requests)]
     out = {"builders": a.serve_slots, "requests": a.serve_requests, "rhs": "loop coil moved per request (seeded)",
            "batches": {}, "batches_note": "most requests in the batched session at one check",
+           "block_solves": {}, "block_note": "block solves that took several queued requests of the system at once",
            "solves_per_s": {}, "iterations_per_s": {}, "latency_ms": {}, "iterations": {},
            "max_true_residual": {}, "setup_s": {}}
     eng = serve.Engine(slots=a.serve_slots)
@@ -187,7 +188,7 @@ def served_run(a, s, code, release):
             t0 = time.perf_counter()
             first = eng.submit(req(rhs[0])).result()   # builds the resident system and factor
             out["setup_s"][name] = time.perf_counter() - t0
-            before = len(eng.batches)
+            before, blocks_before = len(eng.batches), eng.block_solves
             t0 = time.perf_counter()
             futs = eng.submit_many([req(b) for b in rhs])
             resp = [f.result() for f in futs]
@@ -197,6 +198,7 @@ def served_run(a, s, code, release):
                 raise RuntimeError(bad[0])
             out["solves_per_s"][name] = len(resp) / el
             out["batches"][name] = max(eng.batches[before:], default=0)
+            out["block_solves"][name] = eng.block_solves - blocks_before
             out["iterations_per_s"][name] = sum(r.result.iterations for r in resp) / el
             out["latency_ms"][name] = {k: 1e3 * v for k, v in serve.percentiles([r.latency_s for r in resp]).items()}
             out["iterations"][name] = [min(r.result.iterations for r in resp), max(r.result.iterations for r in resp)]

@@ -20,6 +20,11 @@ each with its own HIP stream (a library context):
   others are on the GPU enter the batch without waiting for it to drain.
   IC-MRTR and SGS-MRTR requests (and ICCG with batching off) are solved one
   at a time on the builder that admitted them.
+* Several ICCG requests of one system with the same settings waiting in the
+  queue together are taken by one builder as one block solve
+  (ssgpu_solve_block, up to 32 right-hand sides): one dependency chain per
+  sweep and one pass over the matrix per SpMV for all of them, where the
+  batch would walk a chain per request.
 
 Each result is exactly what a solo solve against the same factor returns
 (the batched kernels use the one-system kernels' blocks and reduction trees;
@@ -98,6 +103,17 @@ def _batch_vector_bytes(n: int, cplx: bool) -> int:
     """Device bytes a system's slot holds in a batch: eight vectors of n
     values (b, x, p, r, z, Ap, sweep scratch, best iterate) and its partials."""
     return (16 if cplx else 8) * n * 8 + 64 * (n // 128 + 1024) + 4096
+
+
+def _block_bytes(n: int, cplx: bool, k: int) -> int:
+    """Upper bound of the work arrays of a block solve of k right-hand sides
+    (ssgpu_solve_block): at most 10 arrays of n x k values (eight of the block
+    iteration, two permuted copies for an ABMC-ordered matrix) and the
+    per-column reduction partials."""
+    return 10 * (16 if cplx else 8) * n * k + 512 * (n // 64 + 1024) + (1 << 20)
+
+
+MAX_BLOCK = 32   # right-hand sides of one block solve (ssgpu_solve_block's limit)
 
 
 def rhs_error(rhs, matrix) -> Optional[str]:
@@ -326,6 +342,7 @@ class Engine:
         self.qcv = threading.Condition()
         self.closing = False
         self.solo_solves = 0
+        self.block_solves = 0        # block solves of several queued requests of one system
         self.runners = {}
         self.contexts = {}           # builder context -> lock guarding its handle
         self._ctx_bytes = {}         # builder context -> last device bytes read from it
@@ -476,6 +493,27 @@ class Engine:
                 return None
             return self.pending.popleft()
 
+    @staticmethod
+    def _block_sig(r: Request):
+        return (r.solver, float(r.accel), int(r.precision), float(r.conv_cri), int(r.max_ite),
+                tuple(sorted(r.opts.items())), r.abmc_block)
+
+    def _coalesce(self, r: Request, key):
+        """Queued requests that can share r's block solve: the same system and
+        settings (up to MAX_BLOCK in all), taken out of the queue in order."""
+        sig = self._block_sig(r)
+        took, keep = [], collections.deque()
+        with self.qcv:
+            while self.pending:
+                item = self.pending.popleft()
+                if (len(took) < MAX_BLOCK - 1 and self._cache_key(item[0]) == key
+                        and self._block_sig(item[0]) == sig):
+                    took.append(item)
+                else:
+                    keep.append(item)
+            self.pending = keep
+        return took
+
     def _reserve_wait(self, nbytes: int, holding: bool, own_ctx=None, patience: float = 2.0):
         """Reserve nbytes, evicting idle systems and trimming idle pools as
         needed, else waiting for memory in use; admitted regardless when
@@ -567,6 +605,48 @@ class Engine:
                     e.matrix.close()
             self.lock.notify_all()
 
+    def _block(self, ctx, stream, key, e: _Entry, items):
+        """One block solve (ssgpu_solve_block) for requests of one system and
+        settings; each column's result equals its solo solve."""
+        torch = self.torch
+        good = []
+        for r, fut, hit in items:
+            err = rhs_error(r.rhs, e.matrix)
+            if err:
+                self._respond(fut, r, None, None, hit, err)
+                self._release(key, e)
+            else:
+                good.append((r, fut, hit))
+        if not good:
+            return
+        r0 = good[0][0]
+        n, cplx = e.matrix.n, e.matrix.dtype == np.complex128
+        need = _block_bytes(n, cplx, len(good))
+        self._reserve_wait(need, holding=True, own_ctx=ctx)
+        try:
+            B = np.stack([np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype) for r, _, _ in good])
+            normB = [float(np.sqrt(np.sum(np.abs(b) ** 2))) for b in B]
+            opts = capi.default_options(accel_factor=r0.accel, factor_precision=r0.precision, **r0.opts)
+            with torch.cuda.stream(stream):
+                d_B = torch.from_numpy(B).to("cuda")
+                d_X = torch.zeros_like(d_B)
+            stream.synchronize()
+            res = e.matrix.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=r0.conv_cri,
+                                       max_ite=r0.max_ite, opts=opts)
+            with torch.cuda.stream(stream):
+                X = d_X.cpu().numpy()
+            with self.lock:
+                self.block_solves += 1
+            for j, (r, fut, hit) in enumerate(good):
+                self._respond(fut, r, X[j], res[j], hit, None)
+        except Exception as ex:  # noqa: BLE001
+            for r, fut, hit in good:
+                self._respond(fut, r, None, None, hit, str(ex))
+        finally:
+            self._unreserve(need)
+            for _ in good:
+                self._release(key, e)
+
     def _worker(self):
         torch = self.torch
         ctx = capi.Context(self.device)
@@ -594,6 +674,15 @@ class Engine:
                 self._release(key, e)
                 continue
             if self.batching and r.solver == capi.ICCG and not r.opts.get("is_diag_scale", 0):
+                # several requests of one system waiting together share one block
+                # solve (one dependency chain and one matrix pass for all of
+                # them); a request alone joins the continuous batch
+                more = self._coalesce(r, key)
+                if more:
+                    with self.lock:
+                        e.users += len(more)
+                    self._block(ctx, stream, key, e, [(r, fut, hit)] + [(q, f, True) for q, f in more])
+                    continue
                 self._runner(e.matrix.dtype == np.complex128, r.precision).put((r, fut, key, e, hit))
                 continue
             pool = _pool_bytes(e.matrix.n, e.matrix.dtype == np.complex128)

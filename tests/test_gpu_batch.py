@@ -15,6 +15,8 @@ from a hipGraph.  Bars:
   replays with host synchronizations between them;
 - every system of a batch that converges meets the tolerance in true residual.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -27,10 +29,17 @@ CONV = 1e-8
 
 
 def _systems(cplx=False):
-    """(system, shift) of uneven sizes; the n=7 system appears with three right-hand sides."""
+    """(system, shift) of uneven sizes; one matrix appears with three right-hand
+    sides (real: the n=7 system, coil moved; complex: the n=6 system with
+    A u right-hand sides), so the batch sweeps them as one group."""
     if cplx:
         specs = [(4, 1, 1.1), (6, 2, 1.2), (5, 3, 1.15), (7, 4, 1.3)]
         out = [(voxel.eddy_current(n, seed=sd), a) for n, sd, a in specs]
+        s6 = out[1][0]
+        A = s6.scipy()
+        for k in range(2):
+            u = np.random.default_rng(60 + k).uniform(-1.0, 1.0, s6.n) * (1.0 + 0.5j)
+            out.append((dataclasses.replace(s6, rhs=A @ u), 1.2))
     else:
         specs = [(4, 1, 1.05), (9, 2, 1.2), (6, 3, 1.3), (12, 4, 1.1), (5, 5, 1.25), (8, 6, 1.15)]
         out = [(voxel.curlcurl(n, seed=sd), a) for n, sd, a in specs]
@@ -311,12 +320,14 @@ def test_batched_mrtr_bit_identical_to_solo(torch_cuda, ctx, cplx, fix):
     solve, including the FP32 factor with its refinement pass (standard
     direction) and per-system options."""
     torch = torch_cuda
-    systems = _systems(cplx)[:5]
+    allsys = _systems(cplx)
+    systems = allsys[:4] + allsys[-2:]   # the last two share a matrix (one sweep group)
     precision = capi.FACTOR_F32 if fix else capi.FACTOR_F64
     S = _Set(torch, ctx, systems, precision)
     try:
         rng = np.random.default_rng(8)
-        cases = [dict(), dict(is_save_best=1), dict(conv_normalize_type=1), dict(is_init=0), dict()]
+        cases = [dict(), dict(is_save_best=1), dict(conv_normalize_type=1), dict(is_init=0), dict(),
+                 dict(is_init=0)]
         x0s = [1e-3 * rng.standard_normal(s.n) * (1 + 1j if cplx else 1) if "is_init" in c else None
                for c, (s, _) in zip(cases, S.items)]
         d_b, d_x = S.vectors(x0s)

@@ -27,13 +27,13 @@ def _rhs(s, seed):
     return s.scipy() @ u
 
 
-def _solo(torch, ctx, s, b, solver, accel, precision=capi.FACTOR_F64):
+def _solo(torch, ctx, s, b, solver, accel, precision=capi.FACTOR_F64, conv_cri=1e-8):
     m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
     d_b = torch.from_numpy(b).cuda()
     d_x = torch.zeros_like(d_b)
     torch.cuda.synchronize()
     o = capi.default_options(accel_factor=accel, factor_precision=precision)
-    r = m.solve_device(solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=5000,
+    r = m.solve_device(solver, d_b.data_ptr(), d_x.data_ptr(), conv_cri=conv_cri, max_ite=5000,
                        normB=float(np.linalg.norm(b)), opts=o)
     ctx.synchronize()
     x = d_x.cpu().numpy()
@@ -198,22 +198,49 @@ def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
 
 
 def test_serving_batches_requests_together(torch_cuda, ctx):
-    """ICCG requests queued together, for one system and for different
-    systems, run in one batched session: the batch holds several of them at
-    once, and each response still equals its solo solve."""
-    systems = [voxel.curlcurl(10, seed=8), voxel.curlcurl(7, seed=9), voxel.curlcurl(12, seed=10)]
-    plan = [(k % 3, _rhs(systems[k % 3], 200 + k)) for k in range(9)]
+    """ICCG requests of different systems queued together run in one batched
+    session: the batch holds several of them at once, and each response still
+    equals its solo solve."""
+    systems = [voxel.curlcurl(n, seed=8 + n) for n in (10, 7, 12, 9, 6, 11)]
+    plan = [(i, _rhs(systems[i], 200 + i)) for i in range(len(systems))]
     eng = serve.Engine(slots=2)
     futs = eng.submit_many([serve.Request(key=i, row_ptr=systems[i].row_ptr, col_idx=systems[i].col_idx,
                                           values=systems[i].values, rhs=b, accel=1.1) for i, b in plan])
     resp = [f.result(timeout=120) for f in futs]
     occupancy = list(eng.batches)
     eng.close()
-    print(f"batch occupancy at the checks: {occupancy}", flush=True)
+    print(f"batch occupancy at the checks: {occupancy}, block solves {eng.block_solves}", flush=True)
     assert max(occupancy) > 1
+    assert eng.block_solves == 0
     for (i, b), r in zip(plan, resp):
         assert r.error is None, r.error
         x_solo, r_solo = _solo(torch_cuda, ctx, systems[i], b, capi.ICCG, 1.1)
+        assert r.result.iterations == r_solo.iterations
+        assert np.array_equal(r.x, x_solo)
+
+
+@pytest.mark.parametrize("cplx,prec", [(False, capi.FACTOR_F64), (False, capi.FACTOR_F32), (True, capi.FACTOR_F64)],
+                         ids=["real", "real-f32", "complex"])
+def test_serving_block_solves_same_system_requests(torch_cuda, ctx, cplx, prec):
+    """Requests of one system with the same settings queued together share a
+    block solve (one chain, one matrix pass for all); a request with other
+    settings does not join it; each response equals its solo solve."""
+    s = voxel.eddy_current(6, seed=4) if cplx else voxel.curlcurl(9, seed=4)
+    bs = [_rhs(s, 300 + k) for k in range(7)]
+    reqs = [serve.Request(key="one", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b, accel=1.1,
+                          precision=prec) for b in bs]
+    reqs.append(serve.Request(key="one", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=bs[0],
+                              accel=1.1, precision=prec, conv_cri=1e-6))
+    eng = serve.Engine(slots=1)
+    first = eng.submit(reqs[0]).result(timeout=120)   # builds the system
+    futs = eng.submit_many(reqs[1:])
+    resp = [first] + [f.result(timeout=120) for f in futs]
+    eng.close()
+    print(f"block solves {eng.block_solves}, batches {list(eng.batches)}", flush=True)
+    assert eng.block_solves >= 1
+    for r, q in zip(resp, reqs):
+        assert r.error is None, r.error
+        x_solo, r_solo = _solo(torch_cuda, ctx, s, q.rhs, capi.ICCG, 1.1, prec, conv_cri=q.conv_cri)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo)
 
