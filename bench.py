@@ -450,6 +450,44 @@ def workload_serving(a):
     print(json.dumps(line), flush=True)
 
 
+def workload_block(a):
+    """32 right-hand sides of the n=95 curl-curl system (the coil moved per
+    column, seeded) as one block solve (ssgpu_solve_block), with the FP64
+    factor and with the FP32 factor: one dependency chain per sweep and one
+    matrix pass per SpMV for all 32 columns.  Reports solves/s and the time
+    per iteration for both factor precisions."""
+    import torch
+    from sparsesolv_amd import capi, voxel
+    s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
+    rhs = np.stack([voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering, coil_seed=k + 1).rhs
+                    for k in range(32)])
+    normB = [float(np.linalg.norm(r)) for r in rhs]
+    ctx = capi.Context(0)
+    out = {"metric": "block solves/s (32 right-hand sides, one factor), FP64 vs FP32 factor", "unit": "solves/s",
+           "n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded voxel curl-curl system)",
+           "config": {"workload": f"shifted ICCG block of 32, curl-curl n={a.n}, N={s.n}, shift {a.accel}, "
+                                  f"tol {a.tol}"}}
+    for name, prec in (("f64", capi.FACTOR_F64), ("f32_factor", capi.FACTOR_F32)):
+        m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
+        m.factorize(capi.ICCG, a.accel, prec)
+        opts = capi.default_options(accel_factor=a.accel, factor_precision=prec)
+        d_B = torch.from_numpy(np.ascontiguousarray(rhs)).cuda()
+        d_X = torch.zeros_like(d_B)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = m.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=a.tol, max_ite=20000, opts=opts)
+        el = time.perf_counter() - t0
+        its = max(r.iterations for r in res)
+        out[name] = {"solves_per_s": 32 / el, "seconds": el, "iterations": its, "ms_per_iteration": 1e3 * el / its,
+                     "converged": sum(bool(r.converged) for r in res),
+                     "max_true_residual": max(r.true_residual for r in res)}
+        print(f"# block {name}: 32 solves in {el:.2f} s ({32 / el:.2f}/s), {its} iterations, "
+              f"{1e3 * el / its:.1f} ms per iteration", file=sys.stderr, flush=True)
+        m.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def workload_eddy(a):
     """Complex symmetric eddy-current system K + j w sigma M on a 64^3 voxel
     grid (sigma = 5.8e7 S/m in a seeded conductor block, w = 2 pi 50), shifted
@@ -551,6 +589,30 @@ def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
     t_spmv = m.time_kernel(0, a.reps, d_b.data_ptr(), d_y.data_ptr())
     t_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
     t_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
+    f32 = None
+    if prec == capi.FACTOR_F64:
+        # the same solve with the FP32 factor: the per-level sweeps here stream
+        # their bytes, so the narrower factor shortens them
+        o32 = capi.default_options(accel_factor=a.accel, factor_precision=capi.FACTOR_F32)
+        d_x.zero_()
+        torch.cuda.synchronize()
+        m.solve_device(code, d_b.data_ptr(), d_x.data_ptr(), conv_cri=a.tol, max_ite=20000, normB=normB, opts=o32)
+        lat32 = []
+        for _ in range(max(1, a.steps)):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r32 = m.solve_device(code, d_b.data_ptr(), d_x.data_ptr(), conv_cri=a.tol, max_ite=20000, normB=normB,
+                                 opts=o32)
+            lat32.append(time.perf_counter() - t)
+        x32 = d_x.cpu().numpy()
+        f32_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=capi.FACTOR_F32)
+        f32_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=capi.FACTOR_F32)
+        f32 = {"solves_per_s": 1.0 / statistics.median(lat32), "ms_per_solve": 1e3 * statistics.median(lat32),
+               "iterations": r32.iterations, "converged": bool(r32.converged),
+               "true_residual": float(np.linalg.norm(s.rhs - s.scipy() @ x32) / normB),
+               "sweep_ms": {"forward": f32_fwd, "backward": f32_bwd}}
+        print(f"# abmc (block {ABMC_BLOCK}), FP32 factor: {f32['ms_per_solve']:.1f} ms/solve, {r32.iterations} "
+              f"iterations, sweeps {f32_fwd:.3f} / {f32_bwd:.3f} ms", file=sys.stderr, flush=True)
     m.close()
     nnzL = (s.nnz + s.n) // 2
     b_spmv, b_sweep = algorithmic_bytes(s.n, s.nnz, nnzL, s_f=4 if prec == capi.FACTOR_F32 else 8)
@@ -566,7 +628,7 @@ def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
             "roofline_sweep": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": gbs / HBM_PEAK_GBS, "traffic": traffic_for(a.n, "sweep_abmc"),
                                "algorithmic_bytes": b_sweep, "avg_ms": sweep_ms},
-            "spmv_ms": t_spmv}
+            "spmv_ms": t_spmv, "f32_factor": f32}
 
 
 def workload_voxelfem(a):
@@ -639,7 +701,8 @@ def main():
                     help="kernels-only: the matrix's own order or its ABMC reordering")
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--workload", default="dropin", choices=["dropin", "batched", "serving", "eddy", "voxelfem"],
+    ap.add_argument("--workload", default="dropin",
+                    choices=["dropin", "batched", "serving", "eddy", "voxelfem", "block"],
                     help="dropin: the n=95 solve (default); batched: 256 systems n=8..40 plus one n=60 system with "
                          "32 right-hand sides; serving: Poisson arrivals of mixed sizes / types")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -667,6 +730,8 @@ def main():
         return workload_serving(a)
     if a.workload == "eddy":
         return workload_eddy(a)
+    if a.workload == "block":
+        return workload_block(a)
     if a.workload == "voxelfem":
         return workload_voxelfem(a)
     t0 = time.perf_counter()
