@@ -11,14 +11,15 @@ each with its own HIP stream (a library context):
   and overlap with the solves already running.  Matrices stay resident with
   their preconditioner in a cache keyed by (matrix key, solver family, shift,
   factor precision, ordering); a request for a cached system skips all that.
-* Batch runners, one per (value type, factor precision), hold a batched ICCG
-  session (ssgpu_batch_*): every ICCG request joins the running batch at the
-  next check (every ``check_every`` iterations), whatever its system; all
+* Batch runners, one per (solver, value type, factor precision), hold a
+  batched ICCG or IC-MRTR session (ssgpu_batch_*): every request joins the
+  running batch of its solver at the next check (every ``check_every``
+  iterations), whatever its system; all
   systems of the batch advance together, one launch per iteration phase for
   all of them, with the iteration replayed from a hipGraph, and each leaves
   as it converges.  This is continuous batching: requests that arrive while
   others are on the GPU enter the batch without waiting for it to drain.
-  IC-MRTR and SGS-MRTR requests (and ICCG with batching off) are solved one
+  SGS-MRTR requests (and every request with batching off) are solved one
   at a time on the builder that admitted them.
 * Several ICCG requests of one system with the same settings waiting in the
   queue together are taken by one builder as one block solve
@@ -99,10 +100,12 @@ def _pool_bytes(n: int, cplx: bool) -> int:
     return 14 * (16 if cplx else 8) * n + (128 << 10)
 
 
-def _batch_vector_bytes(n: int, cplx: bool) -> int:
+def _batch_vector_bytes(n: int, cplx: bool, solver: int = capi.ICCG) -> int:
     """Device bytes a system's slot holds in a batch: eight vectors of n
-    values (b, x, p, r, z, Ap, sweep scratch, best iterate) and its partials."""
-    return (16 if cplx else 8) * n * 8 + 64 * (n // 128 + 1024) + 4096
+    values for ICCG (b, x, p, r, z, Ap, sweep scratch, best iterate), twelve
+    for IC-MRTR (its u, y, w and separate iterate besides), and its partials."""
+    nvec = 8 if solver == capi.ICCG else 12
+    return (16 if cplx else 8) * n * nvec + 64 * (n // 128 + 1024) + 4096
 
 
 def _block_bytes(n: int, cplx: bool, k: int) -> int:
@@ -141,13 +144,14 @@ class _Entry:
 
 
 class _Runner:
-    """One batched ICCG session (one value type and factor precision) on its
-    own context and stream; requests join it between checks."""
+    """One batched session (one solver, ICCG or IC-MRTR, value type and factor
+    precision) on its own context and stream; requests join it between checks."""
 
-    def __init__(self, eng: "Engine", complex_: bool, precision: int):
+    def __init__(self, eng: "Engine", complex_: bool, precision: int, solver: int = capi.ICCG):
         self.eng = eng
         self.complex = complex_
         self.precision = precision
+        self.solver = solver
         self.cv = threading.Condition()
         self.inbox: "collections.deque" = collections.deque()
         self.closing = False
@@ -195,7 +199,7 @@ class _Runner:
             if self.batch is not None:
                 self.graph_fallbacks += self.batch.info()[2]
                 self.batch.close()
-            self.batch = capi.Batch(self.ctx, self.complex, self.precision)
+            self.batch = capi.Batch(self.ctx, self.complex, self.precision, solver=self.solver)
 
     def _run(self):
         eng = self.eng
@@ -261,7 +265,7 @@ class _Runner:
             # result, are reserved from the budget before they are allocated;
             # a request that does not fit waits for the batch to shrink (an
             # empty batch takes it regardless, or nothing would progress)
-            vec = _batch_vector_bytes(n, self.complex)
+            vec = _batch_vector_bytes(n, self.complex, self.solver)
             pool = max(self.pool_hold, _pool_bytes(n, self.complex))
             extra = vec + pool - self.pool_hold
             if not eng._reserve(extra, force=not inflight and not keep):
@@ -478,11 +482,11 @@ class Engine:
         family = "sgs" if r.solver == capi.MRTR_SGS else "ic"
         return (r.key, family, None if family == "sgs" else float(r.accel), int(r.precision), r.abmc_block)
 
-    def _runner(self, complex_: bool, precision: int) -> _Runner:
+    def _runner(self, complex_: bool, precision: int, solver: int = capi.ICCG) -> _Runner:
         with self.lock:
-            k = (complex_, int(precision))
+            k = (complex_, int(precision), int(solver))
             if k not in self.runners:
-                self.runners[k] = _Runner(self, complex_, precision)
+                self.runners[k] = _Runner(self, complex_, precision, solver)
             return self.runners[k]
 
     def _take(self):
@@ -673,17 +677,17 @@ class Engine:
                 self._respond(fut, r, None, None, hit, err)
                 self._release(key, e)
                 continue
-            if self.batching and r.solver == capi.ICCG and not r.opts.get("is_diag_scale", 0):
-                # several requests of one system waiting together share one block
-                # solve (one dependency chain and one matrix pass for all of
-                # them); a request alone joins the continuous batch
-                more = self._coalesce(r, key)
+            if self.batching and r.solver in (capi.ICCG, capi.MRTR_IC) and not r.opts.get("is_diag_scale", 0):
+                # several ICCG requests of one system waiting together share one
+                # block solve (one dependency chain and one matrix pass for all
+                # of them); other requests join the continuous batch of their solver
+                more = self._coalesce(r, key) if r.solver == capi.ICCG else []
                 if more:
                     with self.lock:
                         e.users += len(more)
                     self._block(ctx, stream, key, e, [(r, fut, hit)] + [(q, f, True) for q, f in more])
                     continue
-                self._runner(e.matrix.dtype == np.complex128, r.precision).put((r, fut, key, e, hit))
+                self._runner(e.matrix.dtype == np.complex128, r.precision, r.solver).put((r, fut, key, e, hit))
                 continue
             pool = _pool_bytes(e.matrix.n, e.matrix.dtype == np.complex128)
             self._reserve_wait(pool, holding=True, own_ctx=ctx)

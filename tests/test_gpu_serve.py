@@ -197,24 +197,27 @@ def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
     m.close()
 
 
-def test_serving_batches_requests_together(torch_cuda, ctx):
-    """ICCG requests of different systems queued together run in one batched
-    session: the batch holds several of them at once, and each response still
-    equals its solo solve."""
+@pytest.mark.parametrize("solver", [capi.ICCG, capi.MRTR_IC], ids=["iccg", "mrtr_ic"])
+def test_serving_batches_requests_together(torch_cuda, ctx, solver):
+    """ICCG (IC-MRTR) requests of different systems queued together run in one
+    batched ICCG (IC-MRTR) session: the batch holds several of them at once,
+    and each response still equals its solo solve."""
     systems = [voxel.curlcurl(n, seed=8 + n) for n in (10, 7, 12, 9, 6, 11)]
     plan = [(i, _rhs(systems[i], 200 + i)) for i in range(len(systems))]
     eng = serve.Engine(slots=2)
     futs = eng.submit_many([serve.Request(key=i, row_ptr=systems[i].row_ptr, col_idx=systems[i].col_idx,
-                                          values=systems[i].values, rhs=b, accel=1.1) for i, b in plan])
+                                          values=systems[i].values, rhs=b, accel=1.1, solver=solver)
+                            for i, b in plan])
     resp = [f.result(timeout=120) for f in futs]
     occupancy = list(eng.batches)
     eng.close()
-    print(f"batch occupancy at the checks: {occupancy}, block solves {eng.block_solves}", flush=True)
+    print(f"batch occupancy at the checks: {occupancy}, block solves {eng.block_solves}, solo solves "
+          f"{eng.solo_solves}", flush=True)
     assert max(occupancy) > 1
-    assert eng.block_solves == 0
+    assert eng.block_solves == 0 and eng.solo_solves == 0
     for (i, b), r in zip(plan, resp):
         assert r.error is None, r.error
-        x_solo, r_solo = _solo(torch_cuda, ctx, systems[i], b, capi.ICCG, 1.1)
+        x_solo, r_solo = _solo(torch_cuda, ctx, systems[i], b, solver, 1.1)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo)
 
