@@ -20,11 +20,15 @@ namespace {
 
 template <typename T> using arr = py::array_t<T, py::array::c_style | py::array::forcecast>;
 
+// b is read in place from the numpy buffer (a C-contiguous array of the
+// value type is used as is; anything else is converted once by pybind11); the
+// library uploads it straight from there.  x starts as a copy of x0 (the
+// reference takes x0 by value) and the solve writes it in place.
 template <typename Solver, typename Mat, typename T>
 py::tuple solve_impl(Solver& self, double conv_cri, int max_ite, const Mat& A, arr<T> b, arr<T> x0) {
     const slv_int n = A.getSize();
     if (b.size() != n || x0.size() != n) throw std::invalid_argument("b and x0 must have A.getSize() entries");
-    std::vector<T> bv(b.data(), b.data() + n);
+    const T* bp = b.data();   // b stays referenced by this frame while the GIL is released
     py::array_t<T> x(n);
     std::copy(x0.data(), x0.data() + n, x.mutable_data());
     T* xp = x.mutable_data();
@@ -32,7 +36,7 @@ py::tuple solve_impl(Solver& self, double conv_cri, int max_ite, const Mat& A, a
     {
         py::gil_scoped_release nogil;
         // reference lambda: self.solve(conv_cri, max_ite, A, b, x0) on Eigen vectors (normB = b.norm())
-        ok = self.solve(conv_cri, max_ite, norm_eigen_order(bv.data(), n), A, bv.data(), xp);
+        ok = self.solve(conv_cri, max_ite, norm_eigen_order(bp, n), A, bp, xp);
     }
     return py::make_tuple(ok, x);
 }
