@@ -20,6 +20,7 @@
 // bit-identical to its solo solve against the same factor
 // (ssgpu_solve_factored).
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -56,7 +57,8 @@ struct Batch {
     int dtype = SSGPU_R64;
     int precision = SSGPU_FACTOR_F64;
     long spin_limit = kSweepSpinLimit;
-    int sweep_grid = 1024;
+    int sweep_grid = 0;       // fixed sweep grid (SSGPU_BATCH_GRID); 0: sized to the systems at each rebuild
+    int grid_f = 128, grid_b = 128;   // persistent waves of the forward / backward batched sweep
     bool use_graph = true;
     std::vector<Slot> slots;
     std::vector<BatchSys> h_sys;
@@ -153,14 +155,14 @@ template <typename T> struct Runner {
     void sweeps(int want, int in, int mid, int out) {
         int* tk = B.d_tickets();
         if (B.precision == SSGPU_FACTOR_F32) {
-            launch_batch_sweep<T, F>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
+            launch_batch_sweep<T, F>(s, FWD_DIV_LAST, B.kmax_f, B.grid_f, B.n_ford, B.d_ford.as<int2>(),
                                      tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
-            launch_batch_sweep<T, F>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
+            launch_batch_sweep<T, F>(s, BWD_IC, B.kmax_b, B.grid_b, B.n_bord, B.d_bord.as<int2>(),
                                      tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
         } else {
-            launch_batch_sweep<T, T>(s, FWD_DIV_LAST, B.kmax_f, B.sweep_grid, B.n_ford, B.d_ford.as<int2>(),
+            launch_batch_sweep<T, T>(s, FWD_DIV_LAST, B.kmax_f, B.grid_f, B.n_ford, B.d_ford.as<int2>(),
                                      tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
-            launch_batch_sweep<T, T>(s, BWD_IC, B.kmax_b, B.sweep_grid, B.n_bord, B.d_bord.as<int2>(),
+            launch_batch_sweep<T, T>(s, BWD_IC, B.kmax_b, B.grid_b, B.n_bord, B.d_bord.as<int2>(),
                                      tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
         }
     }
@@ -291,6 +293,22 @@ template <typename T> struct Runner {
             ++active;
         }
         const std::vector<int2> ford = merged_order(fl), bord = merged_order(bl);
+        // Sweep grid: a system keeps about 6 levels of chunks in flight (the
+        // one-system sweep's tuned 128-160 waves at n=95, 24.5 chunks per
+        // level); waves beyond what the systems' level fronts can use only
+        // add polling traffic, which slows every level.
+        auto grid_for = [&](const std::vector<std::pair<int, const std::vector<int>*>>& sl, bool fwd) {
+            if (B.sweep_grid > 0) return B.sweep_grid;
+            double want = 0.0;
+            for (const auto& e : sl) {
+                const Matrix& mm = *B.slots[e.first].m;
+                const int nlev = std::max(1, (fwd ? mm.sfL.lv : mm.sfUic.lv)->nlev);
+                want += std::ceil(6.0 * (double)e.second->size() / nlev);
+            }
+            return std::max(64, std::min(1024, (int)want));
+        };
+        B.grid_f = grid_for(fl, true);
+        B.grid_b = grid_for(bl, false);
         upload_vec(c, B.d_items_mv, imv);
         upload_vec(c, B.d_items_red, ired);
         upload_vec(c, B.d_fin, fin);
@@ -584,7 +602,7 @@ int ssgpu_batch_create(ssgpu_context ctx, int solver, int dtype, int factor_prec
     B.precision = factor_precision;
     if (const char* e = std::getenv("SSGPU_SPIN_LIMIT")) B.spin_limit = std::max(0L, std::atol(e));
     if (const char* e = std::getenv("SSGPU_BATCH_GRAPH")) B.use_graph = std::atoi(e) != 0;
-    if (const char* e = std::getenv("SSGPU_BATCH_GRID")) B.sweep_grid = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("SSGPU_BATCH_GRID")) B.sweep_grid = std::max(0, std::atoi(e));
     SSGPU_HIP(hipSetDevice(B.c->device));
     std::lock_guard<std::mutex> lk(B.c->mu);
     B.d_flags.alloc(B.c, 64);
