@@ -67,7 +67,9 @@ typedef struct {
     int conv_normalize_type;     /* 0 ||b||, 1 ||r0||, 2 conv_normalize_const */
     double conv_normalize_const;
     double accel_factor;         /* IC shift; < 0 selects the automatic shift search */
-    int factor_precision;        /* ssgpu_precision */
+    int factor_precision;        /* ssgpu_precision: FP32 stores the IC factor (SGS-MRTR: the triangles
+                                    of S A S) in FP32 for the sweeps; pivots, SpMV, dots and recurrences
+                                    stay FP64 */
     int mrtr_direction_fix;      /* 0: reference p = u + t*u (MatSolverMRTR_ICver.cpp:302-304);
                                     1: standard MRTR p = u + t*p */
 } ssgpu_options;

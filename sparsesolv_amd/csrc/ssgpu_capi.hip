@@ -569,33 +569,48 @@ struct Solver {
             else sweep<T, T>(m.Uic, BWD_IC, m.Uv.as<T>(), nullptr, in, m.diagD.as<T>(), out);
         }
     }
-    // plain sweeps on the current lower / upper values (SGS): values loaded by sgs_values
+    // plain sweeps on the current lower / upper values (SGS): values loaded by
+    // sgs_values; with an FP32 preparation (o.factor_precision) the sliced-ELL
+    // sweeps read the FP32 copies (rows longer than 64 entries keep FP64)
     template <typename T> void fr(const T* r, T* z) {
+        using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         if (m.sfL.kmax > 0) {
             if (!m.sfL.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
-            sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr,
-                           nullptr, z);
+            if (o.factor_precision == SSGPU_FACTOR_F32)
+                sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r,
+                               nullptr, nullptr, z);
+            else
+                sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r,
+                               nullptr, nullptr, z);
         } else {
             sweep<T, T>(m.L, FWD_DIV_LAST, m.Lv.as<T>(), r, nullptr, nullptr, z);
         }
     }
     template <typename T> void bc(const T* r, T* z) {
+        using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         if (m.sfUs.kmax > 0) {
             if (!m.sfUs.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
-            sf_sweep<T, T>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval.as<T>(), m.sfUs.dval.as<T>(), r,
-                           nullptr, nullptr, z);
+            if (o.factor_precision == SSGPU_FACTOR_F32)
+                sf_sweep<T, F>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval32.as<F>(), m.sfUs.dval32.as<F>(),
+                               r, nullptr, nullptr, z);
+            else
+                sf_sweep<T, T>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval.as<T>(), m.sfUs.dval.as<T>(), r,
+                               nullptr, nullptr, z);
         } else {
             sweep<T, T>(m.Us, BWD_DIV_FIRST, m.Uv.as<T>(), r, nullptr, nullptr, z);
         }
     }
-    // load lower(av) / upper(av) values for the plain sweeps
+    // load lower(av) / upper(av) values for the plain sweeps (and their FP32
+    // copies for an FP32 preparation)
     template <typename T> void sgs_values(const T* av) {
+        const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         if (m.Lv.bytes < sizeof(T) * m.L.nnz) m.Lv.alloc(c, sizeof(T) * m.L.nnz);
         if (m.Uv.bytes < sizeof(T) * m.Us.nnz) m.Uv.alloc(c, sizeof(T) * m.Us.nnz);
         launch_gather<T, T>(s, m.L.nnz, m.L.map.as<int>(), av, m.Lv.as<T>());
         launch_gather<T, T>(s, m.Us.nnz, m.Us.map.as<int>(), av, m.Uv.as<T>());
-        if (m.sfL.kmax > 0) sf_values<T>(m.sfL, av, false, false);
-        if (m.sfUs.kmax > 0) sf_values<T>(m.sfUs, av, false, false);
+        if (m.sfL.kmax > 0) sf_values<T>(m.sfL, av, false, f32);
+        if (m.sfUs.kmax > 0) sf_values<T>(m.sfUs, av, false, f32);
+        m.prep_precision = o.factor_precision;
         m.factor_valid = false;
         m.prep_kind = 2;
     }
@@ -924,11 +939,24 @@ struct Solver {
         launch_negate<T>(s, n, Rd, Y);
         T zeta = one<T>(), zeta_old = one<T>(), eta = Host<T>::zero(), nu = one<T>();
         bool conv = false;
+        // the preconditioned operator (I + L)^-1 S A S (I + U)^-1 on Rd.  FP64:
+        // Eisenstat's splitting, U + fr(Rd - U) with U = bc(Rd), which is
+        // exact only when the triangles are those of S A S.  FP32 triangles:
+        // U = bc(Rd), then fr(S A S U) with the matrix in FP64, so only the
+        // preconditioner is rounded and a singular consistent system stays
+        // consistent (the splitting would apply the FP32-rounded matrix,
+        // whose range differs, and stagnate).
+        const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         for (int it = 0; it < max_ite; ++it) {
             bc<T>(Rd, U);
-            launch_sub<T>(s, n, Rd, U, tmpv);
-            fr<T>(tmpv, ARd);
-            launch_add_inplace<T>(s, n, U, ARd);
+            if (f32) {
+                spmv<T>(av, U, tmpv);
+                fr<T>(tmpv, ARd);
+            } else {
+                launch_sub<T>(s, n, Rd, U, tmpv);
+                fr<T>(tmpv, ARd);
+                launch_add_inplace<T>(s, n, U, ARd);
+            }
             launch_dot3<T>(s, n, ARd, Rd, ARd, it == 0 ? nullptr : Y, partial());
             const double* d = fetch(reduce_blocks(n), 3 * kparts<T>());
             const T ar_r = Host<T>::from(d), ar_ar = Host<T>::from(d + kparts<T>());
@@ -1139,6 +1167,8 @@ int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, do
     bool conv = false;
     if (solver == SSGPU_MRTR_SGS) {
         if (m.prep_kind != 2) throw Error{BAD_ARG, "no SGS preparation: call ssgpu_factorize with SSGPU_MRTR_SGS"};
+        if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the SGS preparation was built without its FP32 copy"};
         T* bs = S.vec<T>(14);
         T* xs = S.vec<T>(10);
         launch_copy<T>(s, n, b, bs);
@@ -1911,7 +1941,6 @@ static int check_opts(const ssgpu_options* o, int solver) {
     if (!o) return SSGPU_BAD_ARG;
     if (solver < SSGPU_ICCG || solver > SSGPU_MRTR_SGS) return SSGPU_BAD_ARG;
     if (o->factor_precision != SSGPU_FACTOR_F64 && o->factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
-    if (solver == SSGPU_MRTR_SGS && o->factor_precision != SSGPU_FACTOR_F64) return SSGPU_UNSUPPORTED;
     return SSGPU_OK;
 }
 
@@ -1944,7 +1973,6 @@ int ssgpu_factorize(ssgpu_matrix mh, int solver, double accel, int factor_precis
                     int* pivot_fixes) {
     if (!mh || solver < SSGPU_ICCG || solver > SSGPU_MRTR_SGS) return SSGPU_BAD_ARG;
     if (factor_precision != SSGPU_FACTOR_F64 && factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
-    if (solver == SSGPU_MRTR_SGS && factor_precision != SSGPU_FACTOR_F64) return SSGPU_UNSUPPORTED;
     SSGPU_GUARD_BEGIN
     Matrix& m = mh->m;
     std::lock_guard<std::mutex> lk(m.ctx->mu);
