@@ -317,7 +317,7 @@ class Matrix:
 
 
 class Batch:
-    """A batched ICCG / COCG or IC-MRTR session on one context (ssgpu_batch_*):
+    """A batched ICCG / COCG, IC-MRTR or SGS-MRTR session on one context (ssgpu_batch_*):
     systems join with add(), advance with step(), and leave through poll()."""
 
     def __init__(self, ctx: Context, complex_: bool = False, precision=FACTOR_F64, solver=ICCG):
@@ -378,7 +378,7 @@ class Batch:
 
 
 def solve_batch(ctx: Context, mats, d_bs, d_xs, normBs, *, conv_cri=1e-8, max_ite=10000, opts=None, solver=ICCG):
-    """One-shot batched ICCG / COCG or IC-MRTR (ssgpu_solve_batch); one Result per system."""
+    """One-shot batched ICCG / COCG, IC-MRTR or SGS-MRTR (ssgpu_solve_batch); one Result per system."""
     k = len(mats)
     o = opts or default_options()
     M = (ctypes.c_void_p * k)(*[m.h.value for m in mats])

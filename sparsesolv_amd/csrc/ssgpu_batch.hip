@@ -1,4 +1,4 @@
-// Batched ICCG / COCG and IC-MRTR: many independent systems (uneven sizes,
+// Batched ICCG / COCG, IC-MRTR and SGS-MRTR: many independent systems (uneven sizes,
 // any mix of matrices, several right-hand sides of one matrix) advanced
 // together, one launch per phase of the iteration for all of them.
 //
@@ -53,7 +53,7 @@ struct Slot {
 
 struct Batch {
     Context* c = nullptr;
-    int solver = SSGPU_ICCG;   // SSGPU_ICCG or SSGPU_MRTR_IC
+    int solver = SSGPU_ICCG;   // SSGPU_ICCG, SSGPU_MRTR_IC or SSGPU_MRTR_SGS
     int dtype = SSGPU_R64;
     int precision = SSGPU_FACTOR_F64;
     long spin_limit = kSweepSpinLimit;
@@ -89,6 +89,10 @@ struct Batch {
 };
 
 template <typename T> constexpr int parts() { return sizeof(T) / sizeof(double); }
+
+// the backward sweep's structure: the IC factor's transpose, or for SGS-MRTR
+// the upper triangle of S A S
+const SfCsr& backward_sf(int solver, const Matrix& m) { return solver == SSGPU_MRTR_SGS ? m.sfUs : m.sfUic; }
 
 // chunk levels of a sweep structure: the level of each chunk's first row
 std::vector<int> chunk_levels(const LevelCsr& lv, int n) {
@@ -152,19 +156,25 @@ template <typename T> struct Runner {
     void finalize(int phase, int want) {
         launch_batch_finalize<T>(s, phase, B.n_fin, fin(), sys(), st(), want, B.d_active(), B.d_tickets(), rec);
     }
+    // one sweep of every system with status `want`: forward (ticket 0) or
+    // backward (ticket 1; IC: the factor's transpose with D, SGS: upper(S A S))
+    void sweep(bool fwd, int want, int in, int out) {
+        int* tk = B.d_tickets() + (fwd ? 0 : 1);
+        const int mode = fwd ? FWD_DIV_LAST : B.solver == SSGPU_MRTR_SGS ? BWD_DIV_FIRST : BWD_IC;
+        const int kmax = fwd ? B.kmax_f : B.kmax_b, grid = fwd ? B.grid_f : B.grid_b;
+        const int norder = fwd ? B.n_ford : B.n_bord;
+        const int2* order = (fwd ? B.d_ford : B.d_bord).as<int2>();
+        if (B.precision == SSGPU_FACTOR_F32)
+            launch_batch_sweep<T, F>(s, mode, kmax, grid, norder, order, tk, B.d_err(), B.spin_limit, sys(), st(),
+                                     want, in, out, 2, rec);
+        else
+            launch_batch_sweep<T, T>(s, mode, kmax, grid, norder, order, tk, B.d_err(), B.spin_limit, sys(), st(),
+                                     want, in, out, 2, rec);
+    }
+    // the IC preconditioner: forward, then backward
     void sweeps(int want, int in, int mid, int out) {
-        int* tk = B.d_tickets();
-        if (B.precision == SSGPU_FACTOR_F32) {
-            launch_batch_sweep<T, F>(s, FWD_DIV_LAST, B.kmax_f, B.grid_f, B.n_ford, B.d_ford.as<int2>(),
-                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
-            launch_batch_sweep<T, F>(s, BWD_IC, B.kmax_b, B.grid_b, B.n_bord, B.d_bord.as<int2>(),
-                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
-        } else {
-            launch_batch_sweep<T, T>(s, FWD_DIV_LAST, B.kmax_f, B.grid_f, B.n_ford, B.d_ford.as<int2>(),
-                                     tk, B.d_err(), B.spin_limit, sys(), st(), want, in, mid, 2, rec);
-            launch_batch_sweep<T, T>(s, BWD_IC, B.kmax_b, B.grid_b, B.n_bord, B.d_bord.as<int2>(),
-                                     tk + 1, B.d_err(), B.spin_limit, sys(), st(), want, mid, out, 2, rec);
-        }
+        sweep(true, want, in, mid);
+        sweep(false, want, mid, out);
     }
     void spmv(int want, int in, int out, int with_dot) {
         launch_batch_spmv<T>(s, B.n_mv, mv(), sys(), st(), want, in, out, with_dot, rec);
@@ -174,7 +184,20 @@ template <typename T> struct Runner {
     // reference's early exits, then
     //   ICCG: z0 = M^-1 r0, p = z0, (r, z)   (MatSolverICCG.cpp:212-251)
     //   IC-MRTR: u = M^-1 r0, y = -r0, z = M^-1 y   (MatSolverMRTR_ICver.cpp:224-262)
+    //   SGS-MRTR: ||b~|| first (the normB of the reference's checks), then
+    //   rd = C^-1 tmp before ||rd||, y = -rd   (MatSolverMRTR_SGSver.cpp:127-168)
     void prologue() {
+        if (B.solver == SSGPU_MRTR_SGS) {
+            vec(BO_NORM2, BS_PENDING, BV_B);
+            finalize(BP_NORMB, BS_PENDING);
+            vec(BO_SENT, BS_PENDING, BV_R, BV_U);
+            sweep(true, BS_PENDING, BV_T, BV_R);
+            vec(BO_NORM2, BS_PENDING, BV_R);
+            finalize(BP_PROLOGUE, BS_PENDING);
+            vec(BO_NEG, BS_PRIMED, BV_R, BV_Y);
+            finalize(BP_ACTIVATE, BS_PRIMED);
+            return;
+        }
         vec(BO_NORM2, BS_PENDING, BV_R);
         finalize(BP_PROLOGUE, BS_PENDING);
         if (B.solver == SSGPU_ICCG) {
@@ -193,10 +216,12 @@ template <typename T> struct Runner {
             finalize(BP_ACTIVATE, BS_PRIMED);
         }
     }
-    // one iteration for every active system: MatSolverICCG.cpp:257-318, or
-    // MatSolverMRTR_ICver.cpp:270-360
+    // one iteration for every active system: MatSolverICCG.cpp:257-318,
+    // MatSolverMRTR_ICver.cpp:270-360, or MatSolverMRTR_SGSver.cpp:182-262
     void iteration() {
-        if (B.solver == SSGPU_ICCG) {
+        if (B.solver == SSGPU_MRTR_SGS) {
+            iteration_sgs();
+        } else if (B.solver == SSGPU_ICCG) {
             spmv(BS_ACTIVE, BV_P, BV_Q, 1);
             finalize(BP_ALPHA, BS_ACTIVE);
             vec(BO_CG_UPDATE, BS_ACTIVE, 0);
@@ -214,6 +239,19 @@ template <typename T> struct Runner {
             finalize(BP_MONITOR, BS_ACTIVE);
             vec(BO_MRTR_UPDATE2, BS_ACTIVE, 0);
         }
+    }
+    // SGS-MRTR, as Solver::sgs_main: u = bc(rd); A rd = u + fr(rd - u) with
+    // FP64 triangles (Eisenstat's splitting), fr(S A S u) with FP32 ones
+    void iteration_sgs() {
+        const int split = B.precision == SSGPU_FACTOR_F32 ? 0 : 1;
+        sweep(false, BS_ACTIVE, BV_R, BV_U);
+        vec(BO_SGS_SUB, BS_ACTIVE, split);
+        if (!split) spmv(BS_ACTIVE, BV_U, BV_T, 0);
+        sweep(true, BS_ACTIVE, BV_T, BV_Q);
+        vec(BO_SGS_DOT3, BS_ACTIVE, split);
+        finalize(BP_MRTR_COEF, BS_ACTIVE);
+        vec(BO_SGS_UPDATE, BS_ACTIVE, 0);
+        finalize(BP_MONITOR, BS_ACTIVE);
     }
 
     // Build the graph of one iteration node by node (hipGraphAddKernelNode in
@@ -273,7 +311,8 @@ template <typename T> struct Runner {
         auto levels_of = [&](Matrix* m) -> std::pair<std::vector<int>, std::vector<int>>& {
             for (auto& e : lev_cache)
                 if (e.first == m) return e.second;
-            lev_cache.emplace_back(m, std::make_pair(chunk_levels(*m->sfL.lv, m->n), chunk_levels(*m->sfUic.lv, m->n)));
+            lev_cache.emplace_back(m, std::make_pair(chunk_levels(*m->sfL.lv, m->n),
+                                                     chunk_levels(*backward_sf(B.solver, *m).lv, m->n)));
             return lev_cache.back().second;
         };
         lev_cache.reserve(ns);
@@ -289,7 +328,7 @@ template <typename T> struct Runner {
             fl.emplace_back(i, &lv.first);
             bl.emplace_back(i, &lv.second);
             kf = std::max(kf, q.m->sfL.kmax);
-            kb = std::max(kb, q.m->sfUic.kmax);
+            kb = std::max(kb, backward_sf(B.solver, *q.m).kmax);
             ++active;
         }
         const std::vector<int2> ford = merged_order(fl), bord = merged_order(bl);
@@ -302,7 +341,7 @@ template <typename T> struct Runner {
             double want = 0.0;
             for (const auto& e : sl) {
                 const Matrix& mm = *B.slots[e.first].m;
-                const int nlev = std::max(1, (fwd ? mm.sfL.lv : mm.sfUic.lv)->nlev);
+                const int nlev = std::max(1, (fwd ? mm.sfL.lv : backward_sf(B.solver, mm).lv)->nlev);
                 want += std::ceil(6.0 * (double)e.second->size() / nlev);
             }
             return std::max(64, std::min(1024, (int)want));
@@ -347,14 +386,23 @@ template <typename T> struct Runner {
         T* x = static_cast<T*>(d.vec[BV_X]);
         const T* b = static_cast<const T*>(d.vec[BV_B]);
         const bool conv = q.status == BS_CONVERGED || q.status == BS_EARLY;
-        // IC-MRTR keeps its iterate apart and returns it at the end, except from
-        // the prologue's early exit (MatSolverMRTR_ICver.cpp:230-240, 362-367)
-        if (B.solver == SSGPU_MRTR_IC && q.status != BS_EARLY)
-            launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_XI]), x);
-        // MatSolverICCG.cpp:315-317: on failure with save-best, the best iterate
-        // (the current x when the last check had just saved it)
-        if (!conv && sl.opt.is_save_best && !q.copy_best)
-            launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_BEST]), x);
+        // on failure with save-best, the best iterate (the current one when the
+        // last check had just saved it): MatSolverICCG.cpp:315-317
+        const bool restore = !conv && sl.opt.is_save_best && !q.copy_best;
+        if (B.solver == SSGPU_MRTR_SGS) {
+            // solve_factored's SGS epilogue: x = s o xi (the scaled iterate,
+            // untouched when the prologue returned early); the true residual
+            // is taken against the caller's b
+            launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[restore ? BV_BEST : BV_XI]), x);
+            launch_mul_scale<T>(s, m.n, m.sc.as<T>(), x);
+            b = static_cast<const T*>(d.vec[BV_Z]);
+        } else {
+            // IC-MRTR keeps its iterate apart and returns it at the end, except
+            // from the prologue's early exit (MatSolverMRTR_ICver.cpp:230-240, 362-367)
+            if (B.solver == SSGPU_MRTR_IC && q.status != BS_EARLY)
+                launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_XI]), x);
+            if (restore) launch_copy<T>(s, m.n, static_cast<const T*>(d.vec[BV_BEST]), x);
+        }
         int refinements = 0, refine_its = 0;
         const double true_res = batch_finish<T>(B.c, m, B.solver, sl.opt, conv, sl.conv_cri, sl.max_ite, sl.normB, b,
                                                 x, refinements, refine_its);
@@ -440,7 +488,9 @@ template <typename T> struct Runner {
         const int nvec = B.solver == SSGPU_ICCG ? kBatchVecsIccg : kBatchVecs;
         if (sl.vecs->bytes < vbytes * nvec) sl.vecs->alloc(c, vbytes * nvec);
         const int rblk = reduce_blocks(n);
-        const size_t pbytes = sizeof(double) * 2 * (size_t)std::max(m.nblk, rblk);
+        // partials: the SpMV's dot writes 1-2 doubles per SpMV block, the
+        // three-dot phases of the MRTR solvers 3-6 per reduction block
+        const size_t pbytes = sizeof(double) * std::max<size_t>(2 * (size_t)m.nblk, 6 * (size_t)rblk);
         if (sl.part->bytes < pbytes) sl.part->alloc(c, pbytes);
         BatchSys d{};
         d.n = n;
@@ -449,7 +499,8 @@ template <typename T> struct Runner {
         d.blk = m.blk.as<int>();
         d.rp = m.rp.as<int>();
         d.ci = m.ci.as<int>();
-        d.val = m.val.p;
+        const bool sgs = B.solver == SSGPU_MRTR_SGS;
+        d.val = sgs ? m.scaledA.p : m.val.p;
         const bool f32 = B.precision == SSGPU_FACTOR_F32;
         d.f_srow = m.sfL.srow;
         d.f_soff = m.sfL.slice_off.as<int>();
@@ -457,24 +508,27 @@ template <typename T> struct Runner {
         d.f_ecol = m.sfL.ecol.as<int>();
         d.f_eval = f32 ? m.sfL.eval32.p : m.sfL.eval.p;
         d.f_dval = f32 ? m.sfL.dval32.p : m.sfL.dval.p;
-        d.b_srow = m.sfUic.srow;
-        d.b_soff = m.sfUic.slice_off.as<int>();
-        d.b_rlen = m.sfUic.rlen.as<int>();
-        d.b_ecol = m.sfUic.ecol.as<int>();
-        d.b_eval = f32 ? m.sfUic.eval32.p : m.sfUic.eval.p;
-        d.diagD = m.diagD.p;
+        const SfCsr& bs = backward_sf(B.solver, m);
+        d.b_srow = bs.srow;
+        d.b_soff = bs.slice_off.as<int>();
+        d.b_rlen = bs.rlen.as<int>();
+        d.b_ecol = bs.ecol.as<int>();
+        d.b_eval = f32 ? bs.eval32.p : bs.eval.p;
+        d.b_dval = sgs ? (f32 ? bs.dval32.p : bs.dval.p) : nullptr;
+        d.diagD = sgs ? nullptr : m.diagD.p;
         char* base = static_cast<char*>(sl.vecs->p);
         for (int k = 0; k < kBatchVecs; ++k) d.vec[k] = k < nvec ? base + vbytes * k : nullptr;
         d.part = sl.part->as<double>();
         sl.own_bx = m.ordered || host_io;
         sl.host_io = host_io;
+        const int bv = sgs ? BV_Z : BV_B;   // the caller's b, in the matrix's order
         if (sl.own_bx) {
             // private b and x in the matrix's order (PermutedIO in the one-system
             // path); host arrays come in through two scratch vectors
             const T* sb = static_cast<const T*>(ub);
             const T* sx = static_cast<const T*>(ux);
             if (host_io) {
-                T* tb = static_cast<T*>(m.ordered ? d.vec[BV_T] : d.vec[BV_B]);
+                T* tb = static_cast<T*>(m.ordered ? d.vec[BV_T] : d.vec[bv]);
                 T* tx = static_cast<T*>(m.ordered ? d.vec[BV_Q] : d.vec[BV_X]);
                 SSGPU_HIP(hipMemcpyAsync(tb, ub, vbytes, hipMemcpyHostToDevice, s));
                 SSGPU_HIP(hipMemcpyAsync(tx, ux, vbytes, hipMemcpyHostToDevice, s));
@@ -482,20 +536,41 @@ template <typename T> struct Runner {
                 sx = tx;
             }
             if (m.ordered) {
-                launch_gather<T, T>(s, n, m.perm.as<int>(), sb, static_cast<T*>(d.vec[BV_B]));
+                launch_gather<T, T>(s, n, m.perm.as<int>(), sb, static_cast<T*>(d.vec[bv]));
                 launch_gather<T, T>(s, n, m.perm.as<int>(), sx, static_cast<T*>(d.vec[BV_X]));
             }
         } else {
-            d.vec[BV_B] = const_cast<void*>(ub);
+            d.vec[bv] = const_cast<void*>(ub);
             d.vec[BV_X] = ux;
         }
-        // the first lines of iccg_main / mrtr_main, on this system alone
+        // the first lines of iccg_main / mrtr_main / sgs_main, on this system alone
         T* x = static_cast<T*>(d.vec[BV_X]);
         const T* b = static_cast<const T*>(d.vec[BV_B]);
         T* r = static_cast<T*>(d.vec[BV_R]);
         T* q = static_cast<T*>(d.vec[BV_Q]);
         if (o.is_save_best) launch_fill<T>(s, n, static_cast<T*>(d.vec[BV_BEST]), T{});
-        if (B.solver == SSGPU_ICCG) {
+        if (sgs) {
+            // solve_factored's scaling: b~ = s o b, xi = x0 / s (MatSolverMRTR_SGSver.cpp:42-69);
+            // then tmp = b~, less S A S xi only when xi is the zero start
+            // (MatSolverMRTR_SGSver.cpp:113-117); ||b~|| and rd = C^-1 tmp come
+            // in the prologue
+            T* bt = static_cast<T*>(d.vec[BV_B]);
+            T* xi = static_cast<T*>(d.vec[BV_XI]);
+            T* t = static_cast<T*>(d.vec[BV_T]);
+            launch_copy<T>(s, n, static_cast<const T*>(d.vec[BV_Z]), bt);
+            launch_mul_scale<T>(s, n, m.sc.as<T>(), bt);
+            launch_fill<T>(s, n, static_cast<T*>(d.vec[BV_P]), T{});
+            if (o.is_init) {
+                launch_fill<T>(s, n, xi, T{});
+                launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.scaledA.as<T>(), xi, q,
+                               nullptr, nullptr, m.spmv_stage_kb);
+                launch_residual<T>(s, n, bt, q, t);
+            } else {
+                launch_copy<T>(s, n, x, xi);
+                launch_div_scale<T>(s, n, m.sc.as<T>(), xi);
+                launch_residual<T>(s, n, bt, nullptr, t);
+            }
+        } else if (B.solver == SSGPU_ICCG) {
             if (o.is_init) launch_fill<T>(s, n, x, T{});
             if (!o.is_init) {
                 launch_spmv<T>(s, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), x, q,
@@ -566,10 +641,16 @@ namespace {
 void check_batch_matrix(const Batch& B, const Matrix& m) {
     if (m.ctx->device != B.c->device) throw Error{BAD_ARG, "matrix and batch are on different devices"};
     if (m.dtype != B.dtype) throw Error{BAD_ARG, "matrix value type differs from the batch's"};
-    if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
-    if (B.precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
-        throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
-    if (m.sfL.kmax <= 0 || m.sfUic.kmax <= 0)
+    if (B.solver == SSGPU_MRTR_SGS) {
+        if (m.prep_kind != 2) throw Error{BAD_ARG, "no SGS preparation: call ssgpu_factorize with SSGPU_MRTR_SGS"};
+        if (B.precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the SGS preparation was built without its FP32 copy"};
+    } else {
+        if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
+        if (B.precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
+    }
+    if (m.sfL.kmax <= 0 || backward_sf(B.solver, m).kmax <= 0)
         throw Error{UNSUPPORTED, "batched solves need rows of at most 64 off-diagonal entries"};
 }
 
@@ -591,8 +672,7 @@ extern "C" {
 int ssgpu_batch_create(ssgpu_context ctx, int solver, int dtype, int factor_precision, ssgpu_batch* out) {
     if (!ctx || !out || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
     if (factor_precision != SSGPU_FACTOR_F64 && factor_precision != SSGPU_FACTOR_F32) return SSGPU_BAD_ARG;
-    if (solver == SSGPU_MRTR_SGS) return SSGPU_UNSUPPORTED;
-    if (solver != SSGPU_ICCG && solver != SSGPU_MRTR_IC) return SSGPU_BAD_ARG;
+    if (solver != SSGPU_ICCG && solver != SSGPU_MRTR_IC && solver != SSGPU_MRTR_SGS) return SSGPU_BAD_ARG;
     SSGPU_GUARD_BEGIN
     std::unique_ptr<ssgpu_batch_s> h(new ssgpu_batch_s());
     Batch& B = h->b;
@@ -617,7 +697,7 @@ int ssgpu_batch_create(ssgpu_context ctx, int solver, int dtype, int factor_prec
 int ssgpu_batch_add(ssgpu_batch bh, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
                     double normB, const void* d_b, void* d_x, int64_t tag) {
     if (!bh || !mh || !opt || !d_b || !d_x) return SSGPU_BAD_ARG;
-    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    if (opt->is_diag_scale && bh->b.solver != SSGPU_MRTR_SGS) return SSGPU_UNSUPPORTED;
     SSGPU_GUARD_BEGIN
     Batch& B = bh->b;
     Matrix& m = mh->m;
@@ -631,7 +711,7 @@ int ssgpu_batch_add(ssgpu_batch bh, ssgpu_matrix mh, const ssgpu_options* opt, d
 int ssgpu_batch_add_host(ssgpu_batch bh, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
                          double normB, const void* b, void* x, int64_t tag) {
     if (!bh || !mh || !opt || !b || !x) return SSGPU_BAD_ARG;
-    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    if (opt->is_diag_scale && bh->b.solver != SSGPU_MRTR_SGS) return SSGPU_UNSUPPORTED;
     SSGPU_GUARD_BEGIN
     Batch& B = bh->b;
     Matrix& m = mh->m;
@@ -722,7 +802,7 @@ int ssgpu_solve_batch(ssgpu_context ctx, int solver, int nsys, const ssgpu_matri
                       double conv_cri, int max_ite, const double* normB, const void* const* d_b, void* const* d_x,
                       ssgpu_result* res) {
     if (!ctx || nsys < 1 || !mats || !opt || !normB || !d_b || !d_x || !res) return SSGPU_BAD_ARG;
-    if (opt->is_diag_scale) return SSGPU_UNSUPPORTED;
+    if (opt->is_diag_scale && solver != SSGPU_MRTR_SGS) return SSGPU_UNSUPPORTED;
     for (int i = 0; i < nsys; ++i)
         if (!mats[i] || !d_b[i] || !d_x[i]) return SSGPU_BAD_ARG;
     const int dtype = mats[0]->m.dtype;

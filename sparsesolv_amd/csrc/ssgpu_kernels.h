@@ -179,7 +179,8 @@ void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta,
 // its solo solve.
 // per-system vectors: ICCG uses the first eight; IC-MRTR also U, Y, W and its
 // internal iterate XI (its x is copied to X at the end); Z holds MRTR's z and Q
-// its v = A u
+// its v = A u.  SGS-MRTR: B the scaled b, Z the caller's b (for the true
+// residual), XI the scaled iterate, R its rd, Q its A rd, T the sweeps' input
 enum BatchVec { BV_B = 0, BV_X, BV_P, BV_R, BV_Z, BV_Q, BV_T, BV_BEST, BV_U, BV_Y, BV_W, BV_XI, kBatchVecs };
 constexpr int kBatchVecsIccg = 8;
 enum BatchStatus {
@@ -197,11 +198,12 @@ struct BatchSys {               // device-resident descriptor of one system
     const void* val;
     const int *f_srow, *f_soff, *f_rlen, *f_ecol;   // forward sweep: sliced-ELL lower(A) with factor values
     const void *f_eval, *f_dval;
-    const int *b_srow, *b_soff, *b_rlen, *b_ecol;   // backward sweep: the factor's transpose
+    const int *b_srow, *b_soff, *b_rlen, *b_ecol;   // backward sweep: the factor's transpose (SGS: upper(S A S))
     const void* b_eval;
+    const void* b_dval;         // SGS: the upper triangle's diagonal (divide-first backward sweep)
     const void* diagD;
     void* vec[kBatchVecs];
-    double* part;               // this system's partial sums (max(nblk, rblk) x 2 doubles)
+    double* part;               // this system's partial sums (max(2 nblk, 6 rblk) doubles)
 };
 struct BatchState {             // Krylov scalars and bookkeeping of one system (device)
     double rz[2], alpha[2], beta[2];   // ICCG: (r, z), alpha, beta; complex as (re, im)
@@ -225,7 +227,10 @@ enum BatchOp {
     BO_NEG,           // vec[b] = -1 * vec[a]
     BO_DOT3,          // partials w.r, w.v, w.y (w.y only after the first iteration), w = vec[W]
     BO_MRTR_UPDATE,   // p = u + t u (or u + t p); xi += zeta p; y = y eta + zeta v; r -= y; partial |r|^2
-    BO_MRTR_UPDATE2   // best = xi when asked; z = z eta + zeta w; u -= z; t, w = sentinel
+    BO_MRTR_UPDATE2,  // best = xi when asked; z = z eta + zeta w; u -= z; t, w = sentinel
+    BO_SGS_SUB,       // best = xi when asked; q = sentinel; a != 0: t = r - u (Eisenstat's splitting)
+    BO_SGS_DOT3,      // a != 0: q += u; partials q.r, q.q, q.y (q.y only after the first iteration)
+    BO_SGS_UPDATE     // p = p t + u; xi += zeta p; y = y eta + zeta q; r -= y; partial |r|^2; u = sentinel
 };
 // scalar phases (one workgroup per system: the fixed-tree finalize of its partials, then)
 enum BatchPhase {
@@ -235,7 +240,9 @@ enum BatchPhase {
     BP_MONITOR,       // ||r||: convergence, best iterate, divergence, max_ite
     BP_BETA,          // beta = rz' / rz; rz = rz'
     BP_ACTIVATE,      // (no sums) PRIMED -> ACTIVE, sweep tickets zeroed
-    BP_MRTR_COEF      // zeta, eta, nu, t from w.r, w.v, w.y (MatSolverMRTR_ICver.cpp:279-300)
+    BP_MRTR_COEF,     // zeta, eta, nu, t from w.r, w.v, w.y (MatSolverMRTR_ICver.cpp:279-300; the same
+                      // recurrence as MatSolverMRTR_SGSver.cpp:196-216)
+    BP_NORMB          // normB = ||vec||, sweep tickets zeroed (SGS-MRTR: the scaled b's norm)
 };
 // A graph built node by node, without stream capture (so no stream is ever
 // in capture mode, whatever other threads do meanwhile): a batched launcher

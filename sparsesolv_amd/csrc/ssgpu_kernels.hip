@@ -636,6 +636,71 @@ __global__ __launch_bounds__(kBS) void k_batch_vec(const int2* __restrict__ item
             w[i] = sv;
             t[i] = sv;
         }
+    } else if (OP == BO_SGS_SUB) {
+        // SGS-MRTR, between the backward and the forward sweep: the best-iterate
+        // copy the monitor asked for, the forward sweep's output q back to the
+        // sentinel, and with a != 0 t = r - u as k_sub (Eisenstat's splitting)
+        if (q.copy_best) {
+            const T* x = bvec<T>(S, BV_XI);
+            T* best = bvec<T>(S, BV_BEST);
+            for (int i = i0; i < n; i += stride) best[i] = x[i];
+        }
+        const T sv = sentinel_value<T>();
+        const T* r = bvec<T>(S, BV_R);
+        const T* u = bvec<T>(S, BV_U);
+        T* t = bvec<T>(S, BV_T);
+        T* v = bvec<T>(S, BV_Q);
+        for (int i = i0; i < n; i += stride) {
+            if (a) t[i] = Num<T>::sub(r[i], u[i]);
+            v[i] = sv;
+        }
+    } else if (OP == BO_SGS_DOT3) {
+        // with a != 0 q += u as k_add_inplace, then k_dot3 (q; r, q, y), y from
+        // the second iteration on (MatSolverMRTR_SGSver.cpp:186-204)
+        const T* u = bvec<T>(S, BV_U);
+        const T* r = bvec<T>(S, BV_R);
+        const T* y = q.iterations > 0 ? bvec<T>(S, BV_Y) : nullptr;
+        T* v = bvec<T>(S, BV_Q);
+        double acc[3 * K] = {};
+        for (int i = i0; i < n; i += stride) {
+            T vi = v[i];
+            if (a) {
+                vi = Num<T>::add(vi, u[i]);
+                v[i] = vi;
+            }
+            Parts<T>::put(acc, Num<T>::mul(vi, r[i]));
+            Parts<T>::put(acc + K, Num<T>::mul(vi, vi));
+            if (y) Parts<T>::put(acc + 2 * K, Num<T>::mul(vi, y[i]));
+        }
+        block_sum<3 * K>(acc);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3 * K; ++k) S.part[it.y * 3 * K + k] = acc[k];
+    } else if (OP == BO_SGS_UPDATE) {
+        // as k_sgs_update (MatSolverMRTR_SGSver.cpp:216-229) on the scaled
+        // iterate xi; the backward sweep's output u goes back to the sentinel
+        // once it is read
+        const T tt = from_parts<T>(q.t), zeta = from_parts<T>(q.zeta), eta = from_parts<T>(q.eta);
+        const T sv = sentinel_value<T>();
+        const T* v = bvec<T>(S, BV_Q);
+        T* u = bvec<T>(S, BV_U);
+        T* p = bvec<T>(S, BV_P);
+        T* x = bvec<T>(S, BV_XI);
+        T* y = bvec<T>(S, BV_Y);
+        T* r = bvec<T>(S, BV_R);
+        double acc[1] = {0.0};
+        for (int i = i0; i < n; i += stride) {
+            const T pi = Num<T>::add(Num<T>::mul(p[i], tt), u[i]);
+            p[i] = pi;
+            u[i] = sv;
+            x[i] = Num<T>::add(x[i], Num<T>::mul(zeta, pi));
+            const T yi = Num<T>::add(Num<T>::mul(y[i], eta), Num<T>::mul(zeta, v[i]));
+            y[i] = yi;
+            const T ri = Num<T>::sub(r[i], yi);
+            r[i] = ri;
+            acc[0] += Num<T>::abs2(ri);
+        }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) S.part[it.y] = acc[0];
     }
 }
 
@@ -648,13 +713,14 @@ template <typename T, int PH>
 __global__ __launch_bounds__(kBS) void k_batch_finalize(const int* __restrict__ fin, const BatchSys* __restrict__ sys,
                                                         BatchState* __restrict__ st, int want, int* __restrict__ active,
                                                         int* __restrict__ tickets) {
-    if ((PH == BP_PROLOGUE || PH == BP_MONITOR || PH == BP_ACTIVATE) && blockIdx.x == 0 && threadIdx.x < 2)
+    if ((PH == BP_PROLOGUE || PH == BP_MONITOR || PH == BP_ACTIVATE || PH == BP_NORMB) && blockIdx.x == 0 &&
+        threadIdx.x < 2)
         tickets[threadIdx.x] = 0;
     const int s = fin[blockIdx.x];
     if (st[s].status != want) return;
     const BatchSys& S = sys[s];
     constexpr int K = Parts<T>::K;
-    constexpr int KP = (PH == BP_PROLOGUE || PH == BP_MONITOR) ? 1 : PH == BP_MRTR_COEF ? 3 * K : K;
+    constexpr int KP = (PH == BP_PROLOGUE || PH == BP_MONITOR || PH == BP_NORMB) ? 1 : PH == BP_MRTR_COEF ? 3 * K : K;
     __shared__ double sum[6];
     if (PH != BP_ACTIVATE) finalize_parts(PH == BP_ALPHA ? S.nblk : S.rblk, KP, S.part, sum);
     if (threadIdx.x != 0) return;
@@ -679,6 +745,8 @@ __global__ __launch_bounds__(kBS) void k_batch_finalize(const int* __restrict__ 
         else if (q.normalize_type == 2) q.normalizer = q.normalize_const;
         else q.normalizer = normB;
         q.status = BS_PRIMED;
+    } else if (PH == BP_NORMB) {
+        q.normB = sqrt(sum[0]);   // as Solver::norm of the scaled b (the status stays pending)
     } else if (PH == BP_RZ0) {
         q.rz[0] = sum[0];
         q.rz[1] = KP > 1 ? sum[1] : 0.0;
@@ -801,6 +869,9 @@ void launch_batch_vec(hipStream_t s, int op, int nitems, const int2* items, cons
         case BO_DOT3: SSGPU_BV(BO_DOT3); break;
         case BO_MRTR_UPDATE: SSGPU_BV(BO_MRTR_UPDATE); break;
         case BO_MRTR_UPDATE2: SSGPU_BV(BO_MRTR_UPDATE2); break;
+        case BO_SGS_SUB: SSGPU_BV(BO_SGS_SUB); break;
+        case BO_SGS_DOT3: SSGPU_BV(BO_SGS_DOT3); break;
+        case BO_SGS_UPDATE: SSGPU_BV(BO_SGS_UPDATE); break;
         case BO_FILL0: SSGPU_BV(BO_FILL0); break;
         case BO_RESID: SSGPU_BV(BO_RESID); break;
         case BO_NORM2: SSGPU_BV(BO_NORM2); break;
@@ -826,6 +897,7 @@ void launch_batch_finalize(hipStream_t s, int phase, int nfin, const int* fin, c
         case BP_BETA: SSGPU_BF(BP_BETA); break;
         case BP_ACTIVATE: SSGPU_BF(BP_ACTIVATE); break;
         case BP_MRTR_COEF: SSGPU_BF(BP_MRTR_COEF); break;
+        case BP_NORMB: SSGPU_BF(BP_NORMB); break;
         default: SSGPU_BF(BP_MONITOR); break;
     }
 #undef SSGPU_BF

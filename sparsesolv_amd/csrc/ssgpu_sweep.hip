@@ -234,6 +234,11 @@ __global__ __launch_bounds__(kSfRows) void k_batch_sweep(int norder, const int2*
                                             static_cast<const F*>(S.f_eval), static_cast<const F*>(S.f_dval),
                                             static_cast<const T*>(S.vec[in]), nullptr, nullptr, max_idle,
                                             static_cast<T*>(S.vec[out]), spins);
+        else if (MODE == BWD_DIV_FIRST)
+            ok = sf_chunk<T, F, MODE, KMAX>(sc.y, S.n, err, spin_limit, S.b_srow, S.b_soff, S.b_rlen, S.b_ecol,
+                                            static_cast<const F*>(S.b_eval), static_cast<const F*>(S.b_dval),
+                                            static_cast<const T*>(S.vec[in]), nullptr, nullptr, max_idle,
+                                            static_cast<T*>(S.vec[out]), spins);
         else
             ok = sf_chunk<T, F, MODE, KMAX>(sc.y, S.n, err, spin_limit, S.b_srow, S.b_soff, S.b_rlen, S.b_ecol,
                                             static_cast<const F*>(S.b_eval), nullptr, nullptr,
@@ -582,9 +587,12 @@ void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder,
     if (mode == FWD_DIV_LAST)
         batch_sweep_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in,
                                              out, max_idle, rec);
-    else
+    else if (mode == BWD_IC)
         batch_sweep_kmax<T, F, BWD_IC>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in, out,
                                        max_idle, rec);
+    else
+        batch_sweep_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, norder, order, ticket, err, spin_limit, sys, st, want, in,
+                                              out, max_idle, rec);
 }
 
 template <typename T>

@@ -12,15 +12,16 @@ each with its own HIP stream (a library context):
   their preconditioner in a cache keyed by (matrix key, solver family, shift,
   factor precision, ordering); a request for a cached system skips all that.
 * Batch runners, one per (solver, value type, factor precision), hold a
-  batched ICCG or IC-MRTR session (ssgpu_batch_*): every request joins the
+  batched ICCG, IC-MRTR or SGS-MRTR session (ssgpu_batch_*): every request joins the
   running batch of its solver at the next check (every ``check_every``
   iterations), whatever its system; all
   systems of the batch advance together, one launch per iteration phase for
   all of them, with the iteration replayed from a hipGraph, and each leaves
   as it converges.  This is continuous batching: requests that arrive while
   others are on the GPU enter the batch without waiting for it to drain.
-  SGS-MRTR requests (and every request with batching off) are solved one
-  at a time on the builder that admitted them.
+  Requests with diagonal scaling on a cached IC factor (which the library
+  refuses there) and every request with batching off are solved one at a
+  time on the builder that admitted them.
 * Several ICCG requests of one system with the same settings waiting in the
   queue together are taken by one builder as one block solve
   (ssgpu_solve_block, up to 32 right-hand sides): one dependency chain per
@@ -677,7 +678,8 @@ class Engine:
                 self._respond(fut, r, None, None, hit, err)
                 self._release(key, e)
                 continue
-            if self.batching and r.solver in (capi.ICCG, capi.MRTR_IC) and not r.opts.get("is_diag_scale", 0):
+            # (SGS-MRTR always scales, whatever is_diag_scale says)
+            if self.batching and (r.solver == capi.MRTR_SGS or not r.opts.get("is_diag_scale", 0)):
                 # several ICCG requests of one system waiting together share one
                 # block solve (one dependency chain and one matrix pass for all
                 # of them); other requests join the continuous batch of their solver

@@ -197,11 +197,11 @@ def test_block_solve_fp32_factor_and_save_best(torch_cuda, ctx):
     m.close()
 
 
-@pytest.mark.parametrize("solver", [capi.ICCG, capi.MRTR_IC], ids=["iccg", "mrtr_ic"])
+@pytest.mark.parametrize("solver", [capi.ICCG, capi.MRTR_IC, capi.MRTR_SGS], ids=["iccg", "mrtr_ic", "mrtr_sgs"])
 def test_serving_batches_requests_together(torch_cuda, ctx, solver):
-    """ICCG (IC-MRTR) requests of different systems queued together run in one
-    batched ICCG (IC-MRTR) session: the batch holds several of them at once,
-    and each response still equals its solo solve."""
+    """Requests of one solver for different systems queued together run in
+    one batched session of that solver: the batch holds several of them at
+    once, and each response still equals its solo solve."""
     systems = [voxel.curlcurl(n, seed=8 + n) for n in (10, 7, 12, 9, 6, 11)]
     plan = [(i, _rhs(systems[i], 200 + i)) for i in range(len(systems))]
     eng = serve.Engine(slots=2)
