@@ -210,7 +210,7 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
                       int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X,
                       ssgpu_result* res);
 
-/* Batched ICCG / COCG, or IC-MRTR, over many independent systems (the caller's own
+/* Batched ICCG / COCG, IC-MRTR or SGS-MRTR over many independent systems (the caller's own
  * matrices, of any sizes; one matrix may appear several times with different
  * right-hand sides).  No counterpart in the reference, which solves one system
  * per call with its Krylov loop on the host (MatSolverICCG.cpp:257-318): here
@@ -220,10 +220,13 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
  * x, iteration count and residuals as ssgpu_solve_factored on it alone (same
  * factor, same options).
  *
- * A batch is a session on one context for one solver (SSGPU_ICCG or
- * SSGPU_MRTR_IC; MatSolverMRTR_ICver.cpp:183-373 with its quirks, or the
- * standard direction with mrtr_direction_fix): systems join with ssgpu_batch_add
- * (the matrix factorized by ssgpu_factorize, no diagonal scaling;
+ * A batch is a session on one context for one solver (SSGPU_ICCG,
+ * SSGPU_MRTR_IC: MatSolverMRTR_ICver.cpp:183-373 with its quirks, or the
+ * standard direction with mrtr_direction_fix; or SSGPU_MRTR_SGS:
+ * MatSolverMRTR_SGSver.cpp:42-282 with S A S from ssgpu_factorize(...,
+ * SSGPU_MRTR_SGS, ...)): systems join with ssgpu_batch_add
+ * (the matrix prepared by ssgpu_factorize for that solver; no diagonal
+ * scaling for the IC solvers, SGS-MRTR always scales;
  * d_b and d_x device vectors of the matrix's n entries that stay valid until
  * the system's result is polled; d_x is the start vector when
  * opt->is_init == 0 and receives the solution) and enter the loop at the
