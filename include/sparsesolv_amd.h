@@ -135,6 +135,31 @@ int ssgpu_matrix_create_ordered(ssgpu_context ctx, int dtype, int32_t n, const i
                                 const int32_t* col_idx, const void* values, int block_size,
                                 ssgpu_matrix* out, int32_t* ncolors);
 
+/* Assemble a matrix of n rows on the device from ntrip (row, col, value)
+ * triplets, taken in the order given, and create it as ssgpu_matrix_create
+ * does.  Replaces the reference's host assembly: mode SSGPU_ASSEMBLE_COO is
+ * the COO constructor SparseMat(n_nonZero, rows, cols, vals)
+ * (SparseMat.cpp:79-104: Eigen's setFromTriplets, duplicates summed in the
+ * order given starting from the first value); SSGPU_ASSEMBLE_BUILDER is
+ * SparseBuilder(n) with one add(row, col, val) per triplet in order, then
+ * SparseMat(builder) (SparseBuilder.hpp:75, SparseBuilder.cpp:107-141,
+ * SparseMat.cpp:113-122: every entry summed from zero in the order given);
+ * SSGPU_ASSEMBLE_BUILDER_SQUARE the same with to_square (an explicit zero at
+ * (n-1, n-1) when no column reaches n-1).  The values are bit-identical to
+ * the reference's.  rows, cols (int32) and values (dtype) are device
+ * pointers when on_device != 0, host arrays otherwise; indices must lie in
+ * [0, n) (SSGPU_BAD_ARG otherwise) and ntrip < 2^31 - 1.  The sort needs
+ * about 40 bytes of device memory per triplet while it runs. */
+typedef enum { SSGPU_ASSEMBLE_COO = 0, SSGPU_ASSEMBLE_BUILDER = 1, SSGPU_ASSEMBLE_BUILDER_SQUARE = 2 } ssgpu_assembly;
+int ssgpu_matrix_create_triplets(ssgpu_context ctx, int dtype, int32_t n, int64_t ntrip, const int32_t* rows,
+                                 const int32_t* cols, const void* values, int mode, int on_device,
+                                 ssgpu_matrix* out);
+/* The matrix as the library stores it (an ABMC-ordered handle holds
+ * P A P^T): row_ptr (n+1), col_idx and values (nnz from ssgpu_matrix_info)
+ * into host arrays; any pointer may be NULL.  The reference's accessors of
+ * the stored matrix (SparseMat.hpp:71-78, getCols SparseMat.cpp:159-179). */
+int ssgpu_matrix_get_csr(ssgpu_matrix m, int32_t* row_ptr, int32_t* col_idx, void* values);
+
 /* Replace the values (same pattern) from host memory.  A factor built from
  * the old values is dropped: cached solves need ssgpu_factorize again. */
 int ssgpu_matrix_update_values(ssgpu_matrix m, const void* values);

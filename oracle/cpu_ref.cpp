@@ -16,6 +16,7 @@
 
 #include "cpu_ref.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <complex>
@@ -763,9 +764,55 @@ int solve_t(int solver, const ref_options* opt, double conv_cri, int max_ite, do
     return r;
 }
 
+// Triplet assembly (SparseMat.cpp:79-104 for mode 0; SparseBuilder.hpp:75,
+// SparseBuilder.cpp:107-141 and SparseMat.cpp:113-122 for modes 1 and 2):
+// entries ordered by (row, col), each entry's triplets summed in the order
+// they were given -- from the first value (Eigen's setFromTriplets), or from
+// a value-initialized zero (the builder's std::map += per add).
+template <typename T>
+long long assemble(int mode, int n, long long ntrip, const int* rows, const int* cols, const T* vals, int* rp,
+                   int* ci, T* v) {
+    for (long long i = 0; i < ntrip; ++i)
+        if (rows[i] < 0 || rows[i] >= n || cols[i] < 0 || cols[i] >= n) return -1;
+    std::vector<long long> ord((size_t)ntrip);
+    for (long long i = 0; i < ntrip; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](long long a, long long b) {
+        return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
+    });
+    for (int i = 0; i <= n; ++i) rp[i] = 0;
+    long long nnz = 0;
+    int maxcol = 0;
+    for (long long k = 0; k < ntrip;) {
+        const int r = rows[ord[k]], c = cols[ord[k]];
+        T s = mode == 0 ? vals[ord[k]] : T{} + vals[ord[k]];
+        long long e = k + 1;
+        while (e < ntrip && rows[ord[e]] == r && cols[ord[e]] == c) s += vals[ord[e++]];
+        ci[nnz] = c;
+        v[nnz] = s;
+        ++nnz;
+        ++rp[r + 1];
+        maxcol = std::max(maxcol, c);
+        k = e;
+    }
+    if (mode == 2 && maxcol + 1 < n) {   // SparseBuilder.cpp:134-139
+        ci[nnz] = n - 1;
+        v[nnz] = T{};
+        ++nnz;
+        ++rp[n];
+    }
+    for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    return nnz;
+}
+
 }  // namespace
 
 extern "C" {
+
+long long ref_assemble(int cplxf, int mode, int n, long long ntrip, const int* rows, const int* cols,
+                       const double* vals, int* rp, int* ci, double* v) {
+    if (cplxf) return assemble<cplx>(mode, n, ntrip, rows, cols, as<cplx>(vals), rp, ci, as<cplx>(v));
+    return assemble<double>(mode, n, ntrip, rows, cols, vals, rp, ci, v);
+}
 
 void ref_set_reduction_order(int mode) { g_red_mode = mode; }
 

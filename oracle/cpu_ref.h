@@ -98,6 +98,17 @@ int ref_solve(int solver, int cplx, const ref_options* opt, double conv_cri, int
               int n, const int* rp, const int* ci, const double* v, const double* b, double* x,
               double* log, int log_cap, int* log_len);
 
+/* CSR of n rows from ntrip (row, col, value) triplets taken in the order
+ * given.  mode 0: SparseMat(n_nonZero, rows, cols, vals) (SparseMat.cpp:79-104,
+ * Eigen's setFromTriplets: duplicates summed in order from the first value);
+ * mode 1: SparseBuilder::add then SparseMat::build (SparseBuilder.hpp:75,
+ * SparseBuilder.cpp:107-141, SparseMat.cpp:113-122: each entry summed from
+ * zero in order); mode 2: the same with to_square (an explicit zero at
+ * (n-1, n-1) when no column reaches n-1).  rp: n+1; ci, v: ntrip+1 entries.
+ * Returns nnz, or -1 when an index is outside [0, n). */
+long long ref_assemble(int cplx, int mode, int n, long long ntrip, const int* rows, const int* cols,
+                       const double* vals, int* rp, int* ci, double* v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,8 @@ def oracle():
         lib.ref_bc.argtypes = [I, I, V, V, V, V, V]
         lib.ref_diag_scaling.argtypes = [I, I, V, V, V, V, V, V]
         lib.ref_set_reduction_order.argtypes = [I]
+        lib.ref_assemble.argtypes = [I, I, I, ctypes.c_longlong, V, V, V, V, V, V]
+        lib.ref_assemble.restype = ctypes.c_longlong
         lib.ref_solve.argtypes = [I, I, ctypes.POINTER(Options), D, I, D, I, V, V, V, V, V, V, I,
                                   ctypes.POINTER(I)]
         _ORACLE = lib
@@ -106,6 +108,8 @@ def reference():
         lib.refdrv_dot.argtypes = [I, I, V, V, V]
         lib.refdrv_norm.argtypes = [I, I, V]
         lib.refdrv_norm.restype = D
+        lib.refdrv_assemble.argtypes = [I, I, I, ctypes.c_longlong, V, V, V, V, V, V]
+        lib.refdrv_assemble.restype = ctypes.c_longlong
         _REF = lib
     return _REF
 
@@ -254,3 +258,35 @@ def ref_dot(a, b):
 
 def ref_norm(a):
     return reference().refdrv_norm(_cx(a), len(a), _p(a))
+
+
+# ---- triplet assembly ----------------------------------------------------------
+ASSEMBLE_COO, ASSEMBLE_BUILDER, ASSEMBLE_BUILDER_SQUARE = 0, 1, 2
+
+
+def _assemble_with(fn, n, rows, cols, vals, mode, nrows=None):
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.complex128 if np.iscomplexobj(vals) else np.float64)
+    nt = len(rows)
+    m = n if nrows is None else nrows
+    rp = np.zeros(m + 1, np.int32)
+    ci = np.zeros(nt + 1, np.int32)
+    v = np.zeros(nt + 1, vals.dtype)
+    nnz = fn(_cx(vals), int(mode), int(n), nt, _p(rows), _p(cols), _p(vals), _p(rp), _p(ci), _p(v))
+    if nnz < 0:
+        raise ValueError("triplet index outside the matrix")
+    return rp, ci[:nnz].copy(), v[:nnz].copy()
+
+
+def assemble(n, rows, cols, vals, mode=ASSEMBLE_COO):
+    """(row_ptr, col_idx, values) of the n-row matrix the reference builds from
+    these triplets (cpu_ref.cpp restatement)."""
+    return _assemble_with(oracle().ref_assemble, n, rows, cols, vals, mode)
+
+
+def ref_assemble(n, rows, cols, vals, mode=ASSEMBLE_COO):
+    """The same through the compiled reference's SparseMat / SparseBuilder
+    (mode 0 sizes the matrix from the largest row index, as the reference)."""
+    nrows = int(np.max(rows)) + 1 if mode == ASSEMBLE_COO and len(rows) else n
+    return _assemble_with(reference().refdrv_assemble, n, rows, cols, vals, mode, nrows)

@@ -6,12 +6,14 @@
 // C-ABI so tests can check the oracle restatement against the real thing.
 // Nothing here is part of the product.
 #include <SparseSolv/MatSolverICCG.hpp>
+#include <SparseSolv/SparseBuilder.hpp>
 #include <SparseSolv/MatSolverMRTR.hpp>
 #include <SparseSolv/SprsPreconditioner.hpp>
 #include <SparseSolv/SprsTrans.hpp>
 
 #include <complex>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "cpu_ref.h"
@@ -151,6 +153,32 @@ void refdrv_dot(int cplx, int n, const double* a, const double* b, double* out) 
         for (int i = 0; i < n; ++i) { ea(i) = a[i]; eb(i) = b[i]; }
         out[0] = ea.dot(eb);
     }
+}
+
+// The reference's own assembly: the COO constructor (mode 0), or
+// SparseBuilder::add in the order given and SparseMat(builder, to_square)
+// (mode 1: to_square false, mode 2: true).  Returns nnz.
+long long refdrv_assemble(int cplx, int mode, int n, long long ntrip, const int* rows, const int* cols,
+                          const double* vals, int* rp, int* ci, double* v) {
+    auto run = [&](auto tag, auto* mat_tag) -> long long {
+        using T = decltype(tag);
+        using M = typename std::remove_pointer<decltype(mat_tag)>::type;
+        const T* tv = reinterpret_cast<const T*>(vals);
+        if (mode == 0) {
+            std::vector<int> r(rows, rows + ntrip), c(cols, cols + ntrip);
+            std::vector<T> x(tv, tv + ntrip);
+            M A((int)ntrip, r, c, x);
+            csr_out<T>(A, rp, ci, v);
+            return (long long)A.mat().nonZeros();
+        }
+        SparseBuilderTMPL<T> b(n);
+        for (long long i = 0; i < ntrip; ++i) b.add(rows[i], cols[i], tv[i]);
+        M A(b, mode == 2);
+        csr_out<T>(A, rp, ci, v);
+        return (long long)A.mat().nonZeros();
+    };
+    if (cplx) return run(dcomplex{}, (SparseMatC*)nullptr);
+    return run(double{}, (SparseMat*)nullptr);
 }
 
 double refdrv_norm(int cplx, int n, const double* a) {

@@ -37,7 +37,7 @@ def _run(cmd):
 
 
 HIP_SOURCES = ("ssgpu_kernels.hip", "ssgpu_sweep.hip", "ssgpu_icfact.hip", "ssgpu_block.hip", "ssgpu_batch.hip",
-               "ssgpu_order.hip", "ssgpu_capi.hip")
+               "ssgpu_order.hip", "ssgpu_assemble.hip", "ssgpu_capi.hip")
 
 
 def build_hip():

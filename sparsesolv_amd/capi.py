@@ -20,6 +20,7 @@ OK, NOT_CONVERGED, NO_DIAGONAL, BAD_ARG, OOM, HIP_ERROR, NOT_SQUARE, UNSUPPORTED
 R64, C128 = 0, 1
 ICCG, MRTR_IC, MRTR_SGS = 0, 1, 2
 FACTOR_F64, FACTOR_F32 = 0, 1
+ASSEMBLE_COO, ASSEMBLE_BUILDER, ASSEMBLE_BUILDER_SQUARE = 0, 1, 2
 
 
 class SolverError(RuntimeError):
@@ -73,6 +74,8 @@ def lib():
         L.ssgpu_matrix_create.argtypes = [V, I, ctypes.c_int32, V, V, V, P(V)]
         L.ssgpu_matrix_create_ordered.argtypes = [V, I, ctypes.c_int32, V, V, V, I, P(V), P(ctypes.c_int32)]
         L.ssgpu_abmc_order.argtypes = [ctypes.c_int32, V, V, I, V, P(ctypes.c_int32)]
+        L.ssgpu_matrix_create_triplets.argtypes = [V, I, ctypes.c_int32, ctypes.c_int64, V, V, V, I, I, P(V)]
+        L.ssgpu_matrix_get_csr.argtypes = [V, V, V, V]
         L.ssgpu_matrix_update_values.argtypes = [V, V]
         L.ssgpu_matrix_destroy.argtypes = [V]
         L.ssgpu_matrix_info.argtypes = [V, P(ctypes.c_int32), P(ctypes.c_int64), P(I), P(ctypes.c_int32),
@@ -198,6 +201,49 @@ class Matrix:
                                                     self.col_idx.ctypes.data, vals.ctypes.data, int(abmc_block),
                                                     ctypes.byref(self.h), ctypes.byref(nc)))
             self.ncolors = nc.value
+
+    @classmethod
+    def from_triplets(cls, ctx: Context, n: int, rows, cols, values, *, mode=ASSEMBLE_COO, ntrip=None,
+                      complex_=None):
+        """Assemble on the GPU from (row, col, value) triplets taken in the
+        order given (ssgpu_matrix_create_triplets): mode ASSEMBLE_COO is the
+        reference's COO constructor, ASSEMBLE_BUILDER[_SQUARE] its
+        SparseBuilder + SparseMat(builder[, to_square]).  rows / cols /
+        values are host arrays, or device pointers (ints) with ntrip and
+        complex_ given."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        self.ncolors = None
+        self.n = int(n)
+        if isinstance(rows, int):
+            if ntrip is None or complex_ is None:
+                raise ValueError("device triplets need ntrip and complex_")
+            self.dtype = np.complex128 if complex_ else np.float64
+            check(lib().ssgpu_matrix_create_triplets(ctx.h, C128 if complex_ else R64, self.n, int(ntrip), rows,
+                                                     cols, values, int(mode), 1, ctypes.byref(self.h)))
+        else:
+            r = np.ascontiguousarray(rows, dtype=np.int32)
+            c = np.ascontiguousarray(cols, dtype=np.int32)
+            self.dtype = np.complex128 if np.iscomplexobj(values) else np.float64
+            v = np.ascontiguousarray(values, dtype=self.dtype)
+            if not (len(r) == len(c) == len(v)):
+                raise ValueError("rows, cols and values differ in length")
+            check(lib().ssgpu_matrix_create_triplets(ctx.h, _dtype_code(v), self.n, len(r), r.ctypes.data,
+                                                     c.ctypes.data, v.ctypes.data, int(mode), 0,
+                                                     ctypes.byref(self.h)))
+        self.row_ptr, self.col_idx, _ = self.csr(values=False)
+        return self
+
+    def csr(self, values=True):
+        """(row_ptr, col_idx, values) as the library stores the matrix (values None when not asked)."""
+        nnz = self.info()["nnz"]
+        rp = np.empty(self.n + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        v = np.empty(nnz, self.dtype) if values else None
+        check(lib().ssgpu_matrix_get_csr(self.h, rp.ctypes.data, ci.ctypes.data,
+                                         v.ctypes.data if values else None))
+        return rp, ci, v
 
     def close(self):
         if self.h:

@@ -1780,6 +1780,21 @@ int ssgpu_memory(int64_t* resident, int64_t* peak, int reset_peak) {
     return SSGPU_OK;
 }
 
+// the common tail of matrix creation (ctx's mutex held): the structural
+// analysis, the SpMV tuner, the handle
+static void finish_create(ssgpu_context ctx, std::unique_ptr<ssgpu_matrix_s>& h, ssgpu_matrix* out) {
+    Matrix& m = h->m;
+    analyse(m);
+    if (m.dtype == SSGPU_R64) tune_spmv<double>(m, 1.0);
+    else tune_spmv<zc>(m, zc{1.0, 0.0});
+    SSGPU_HIP(hipStreamSynchronize(ctx->c.stream));
+    {
+        std::lock_guard<std::mutex> lk(g_lifetime_mu);
+        ++ctx->c.live_matrices;
+    }
+    *out = h.release();
+}
+
 int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                         const void* values, ssgpu_matrix* out) {
     if (!ctx || !out || !row_ptr || n <= 0 || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
@@ -1810,15 +1825,59 @@ int ssgpu_matrix_create(ssgpu_context ctx, int dtype, int32_t n, const int32_t* 
     upload(m.ctx, m.ci, m.h_ci);
     m.val.alloc(m.ctx, m.vsize * (size_t)std::max<int64_t>(m.nnz, 1));
     if (m.nnz) SSGPU_HIP(hipMemcpyAsync(m.val.p, values, m.vsize * m.nnz, hipMemcpyHostToDevice, ctx->c.stream));
-    analyse(m);
-    if (m.dtype == SSGPU_R64) tune_spmv<double>(m, 1.0);
-    else tune_spmv<zc>(m, zc{1.0, 0.0});
-    SSGPU_HIP(hipStreamSynchronize(ctx->c.stream));
-    {
-        std::lock_guard<std::mutex> lk(g_lifetime_mu);
-        ++ctx->c.live_matrices;
+    finish_create(ctx, h, out);
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_matrix_create_triplets(ssgpu_context ctx, int dtype, int32_t n, int64_t ntrip, const int32_t* rows,
+                                 const int32_t* cols, const void* values, int mode, int on_device,
+                                 ssgpu_matrix* out) {
+    if (!ctx || !out || n <= 0 || ntrip < 0 || (dtype != SSGPU_R64 && dtype != SSGPU_C128)) return SSGPU_BAD_ARG;
+    if (mode < SSGPU_ASSEMBLE_COO || mode > SSGPU_ASSEMBLE_BUILDER_SQUARE) return SSGPU_BAD_ARG;
+    if (ntrip > 0 && (!rows || !cols || !values)) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    std::unique_ptr<ssgpu_matrix_s> h(new ssgpu_matrix_s());
+    Matrix& m = h->m;
+    m.ctx = &ctx->c;
+    m.dtype = dtype;
+    m.n = n;
+    m.vsize = dtype == SSGPU_R64 ? 8 : 16;
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    SSGPU_HIP(hipSetDevice(ctx->c.device));
+    DevBuf dr, dc, dv;   // host triplets are staged in device memory first
+    if (!on_device && ntrip > 0) {
+        dr.alloc(m.ctx, sizeof(int32_t) * (size_t)ntrip);
+        dc.alloc(m.ctx, sizeof(int32_t) * (size_t)ntrip);
+        dv.alloc(m.ctx, m.vsize * (size_t)ntrip);
+        SSGPU_HIP(hipMemcpy(dr.p, rows, sizeof(int32_t) * (size_t)ntrip, hipMemcpyHostToDevice));
+        SSGPU_HIP(hipMemcpy(dc.p, cols, sizeof(int32_t) * (size_t)ntrip, hipMemcpyHostToDevice));
+        SSGPU_HIP(hipMemcpy(dv.p, values, m.vsize * (size_t)ntrip, hipMemcpyHostToDevice));
+        rows = dr.as<int32_t>();
+        cols = dc.as<int32_t>();
+        values = dv.p;
     }
-    *out = h.release();
+    assemble_triplets(m, ntrip, rows, cols, values, mode);
+    dr.release();
+    dc.release();
+    dv.release();
+    finish_create(ctx, h, out);
+    return SSGPU_OK;
+    SSGPU_GUARD_END
+}
+
+int ssgpu_matrix_get_csr(ssgpu_matrix mh, int32_t* row_ptr, int32_t* col_idx, void* values) {
+    if (!mh) return SSGPU_BAD_ARG;
+    SSGPU_GUARD_BEGIN
+    Matrix& m = mh->m;
+    std::lock_guard<std::mutex> lk(m.ctx->mu);
+    SSGPU_HIP(hipSetDevice(m.ctx->device));
+    if (row_ptr) std::memcpy(row_ptr, m.h_rp.data(), sizeof(int32_t) * ((size_t)m.n + 1));
+    if (col_idx && m.nnz) std::memcpy(col_idx, m.h_ci.data(), sizeof(int32_t) * (size_t)m.nnz);
+    if (values && m.nnz) {
+        SSGPU_HIP(hipMemcpyAsync(values, m.val.p, m.vsize * (size_t)m.nnz, hipMemcpyDeviceToHost, m.ctx->stream));
+        SSGPU_HIP(hipStreamSynchronize(m.ctx->stream));
+    }
     return SSGPU_OK;
     SSGPU_GUARD_END
 }

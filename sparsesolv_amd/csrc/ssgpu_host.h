@@ -164,6 +164,12 @@ struct Matrix {
     std::vector<int64_t> vmap;   // stored entry q of P A P^T = caller's entry vmap[q]
 };
 
+// CSR of m.n rows assembled on the device from ntrip device triplets in the
+// order given (ssgpu_assemble.hip): fills m.rp, m.ci, m.val, m.h_rp, m.h_ci and
+// m.nnz.  mode 0: duplicates summed from the first value; 1: from zero; 2: as
+// 1 with the to_square padding entry.
+void assemble_triplets(Matrix& m, int64_t ntrip, const int* rows, const int* cols, const void* vals, int mode);
+
 // true residual (and FP32-factor refinement) of a finished batched solve
 template <typename T>
 double batch_finish(Context* c, Matrix& m, int solver, const ssgpu_options& o, bool conv, double conv_cri,
