@@ -1,20 +1,23 @@
 """Serving loop for shifted-IC Krylov solves on one GPU.
 
-Requests (a matrix key, its CSR arrays, a right-hand side, the solver and its
-settings) are admitted as they arrive.  Two kinds of threads share the GPU,
-each with its own HIP stream (a library context):
+Requests (a matrix key, its CSR arrays or its assembly triplets -- element
+data in the reference's add order, see assembly.element_triplets --, a
+right-hand side, the solver and its settings) are admitted as they arrive.
+Two kinds of threads share the GPU, each with its own HIP stream (a library
+context):
 
 * Builders (``slots`` of them) take requests in arrival order.  A request
-  for a system not resident yet uploads the matrix, runs the structural
+  for a system not resident yet uploads the matrix (or assembles its
+  triplets on the GPU, ssgpu_matrix_create_triplets), runs the structural
   analysis and builds the preconditioner (the shifted IC(0) factor, or
   S A S for SGS-MRTR) on the builder's stream, so misses build in parallel
   and overlap with the solves already running.  Matrices stay resident with
   their preconditioner in a cache keyed by (matrix key, solver family, shift,
   factor precision, ordering); a request for a cached system skips all that.
 * Batch runners, one per (solver, value type, factor precision), hold a
-  batched ICCG, IC-MRTR or SGS-MRTR session (ssgpu_batch_*): every request joins the
-  running batch of its solver at the next check (every ``check_every``
-  iterations), whatever its system; all
+  batched ICCG, IC-MRTR or SGS-MRTR session (ssgpu_batch_*): every request
+  joins the running batch of its solver at the next check (every
+  ``check_every`` iterations), whatever its system; all
   systems of the batch advance together, one launch per iteration phase for
   all of them, with the iteration replayed from a hipGraph, and each leaves
   as it converges.  This is continuous batching: requests that arrive while
@@ -59,10 +62,10 @@ from . import capi
 @dataclass
 class Request:
     key: object                      # identifies the matrix (same key => same CSR)
-    row_ptr: np.ndarray
-    col_idx: np.ndarray
-    values: np.ndarray
-    rhs: np.ndarray
+    row_ptr: Optional[np.ndarray] = None
+    col_idx: Optional[np.ndarray] = None
+    values: Optional[np.ndarray] = None
+    rhs: Optional[np.ndarray] = None
     solver: int = capi.ICCG
     accel: float = 1.05
     precision: int = capi.FACTOR_F64
@@ -71,6 +74,20 @@ class Request:
     opts: dict = field(default_factory=dict)
     abmc_block: Optional[int] = None  # solve on the ABMC reordering (not a parity mode; see capi.Matrix)
     arrival: float = 0.0             # perf_counter() at submission (set by submit)
+    # instead of the CSR: (rows, cols, values) triplets in add order (for
+    # element data, assembly.element_triplets), assembled on the GPU when the
+    # system is built; size = the matrix's rows (default: largest index + 1)
+    triplets: Optional[tuple] = None
+    size: Optional[int] = None
+    assemble: int = capi.ASSEMBLE_BUILDER
+
+    def shape(self):
+        """(rows, stored-entry bound, complex) of the request's matrix."""
+        if self.triplets is not None:
+            r, c, v = self.triplets
+            n = self.size if self.size is not None else int(max(np.max(r), np.max(c))) + 1 if len(r) else 1
+            return n, len(r) + 1, np.iscomplexobj(v)
+        return len(self.row_ptr) - 1, len(self.values), np.iscomplexobj(self.values)
 
 
 @dataclass
@@ -99,6 +116,13 @@ def _pool_bytes(n: int, cplx: bool) -> int:
     copy and refinement, plus the two permuted copies of an ABMC-ordered
     matrix) and the reduction partials."""
     return 14 * (16 if cplx else 8) * n + (128 << 10)
+
+
+def _assembly_bytes(ntrip: int, cplx: bool) -> int:
+    """Upper bound of the temporaries of a GPU assembly of ntrip host
+    triplets: their device copy (index pair and value), the sort's keys,
+    payloads and scratch, the entry heads and their scan."""
+    return (72 + (16 if cplx else 8)) * ntrip + (4 << 20)
 
 
 def _batch_vector_bytes(n: int, cplx: bool, solver: int = capi.ICCG) -> int:
@@ -559,8 +583,10 @@ class Engine:
         if needed, so misses in different builders build in parallel), with a
         user reference."""
         key = self._cache_key(r)
-        cplx = np.iscomplexobj(r.values)
-        need = _system_bytes(len(r.values), len(r.row_ptr) - 1, cplx, r.precision)
+        n, nnz, cplx = r.shape()
+        need = _system_bytes(nnz, n, cplx, r.precision)
+        if r.triplets is not None:
+            need += _assembly_bytes(len(r.triplets[0]), cplx)
         if need > self.budget:
             raise MemoryError(f"system needs {need} B, budget {self.budget} B")
         with self.lock:
@@ -587,7 +613,14 @@ class Engine:
             hit = True
         if not hit:
             try:
-                m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values, abmc_block=r.abmc_block)
+                if r.triplets is not None:
+                    m = capi.Matrix.from_triplets(ctx, n, *r.triplets, mode=r.assemble)
+                    if r.abmc_block is not None:   # the ordered handle is built from the assembled CSR
+                        csr = m.csr()
+                        m.close()
+                        m = capi.Matrix(ctx, *csr, abmc_block=r.abmc_block)
+                else:
+                    m = capi.Matrix(ctx, r.row_ptr, r.col_idx, r.values, abmc_block=r.abmc_block)
                 m.factorize(r.solver, r.accel, r.precision)
                 e.matrix = m
             except Exception as ex:   # noqa: BLE001 - reported to every waiter

@@ -6,14 +6,17 @@
   each returns alone (work space is per context; the factor is only read).
 * The serving engine: every response equals the solo solve of its request,
   in a mix of real / complex systems, repeated matrices (factor cache hits)
-  and an HBM budget small enough to force evictions.
+  and an HBM budget small enough to force evictions; requests that carry
+  element data are assembled on the GPU and answer as the oracle-assembled
+  CSR does.
 """
 import threading
 
 import numpy as np
 import pytest
 
-from sparsesolv_amd import capi, serve, voxel
+from oracle import oracle as O
+from sparsesolv_amd import assembly, capi, serve, voxel
 
 pytestmark = pytest.mark.gpu
 
@@ -358,3 +361,55 @@ def test_serving_abmc_requests(torch_cuda, ctx):
     m.close()
     assert resp[1].result.iterations == r.iterations
     assert np.array_equal(resp[1].x, x_abmc)
+
+
+def _element_system(nx, seed, cplx=False):
+    """Element data of an nx^3 voxel mesh (boundary edges eliminated, SPD
+    element matrices; complex: a small imaginary shift) as add-order triplets."""
+    rng = np.random.default_rng(seed)
+    ax = np.linspace(0.0, 1.0, nx + 1)
+    mesh = voxel.tensor_mesh(ax, ax, ax)
+    mid = mesh.edge_midpoints()
+    zero = np.any((mid < 1e-9) | (mid > 1.0 - 1e-9), axis=1)
+    conn = np.where(zero, -1, np.cumsum(~zero) - 1)[mesh.elem_edges]
+    B = rng.standard_normal((len(conn), 12, 12))
+    ke = np.einsum("eki,ekj->eij", B, B) / 12.0 + 0.5 * np.eye(12)
+    if cplx:
+        ke = ke + 0.2j * np.eye(12)
+    return int((~zero).sum()), assembly.element_triplets(conn, ke)
+
+
+def test_serving_element_data_requests(torch_cuda, ctx):
+    """Requests carrying element data (triplets, no CSR) are assembled on the
+    GPU when their system is built and cached like CSR requests; each
+    response equals the solo solve on the oracle-assembled CSR."""
+    specs = [("e6", 6, 1, False), ("e8", 8, 2, False), ("c6", 6, 3, True)]
+    systems = {k: _element_system(nx, sd, cx) for k, nx, sd, cx in specs}
+    plan = []
+    for k in range(9):
+        key = specs[k % 3][0]
+        n, trip = systems[key]
+        rng = np.random.default_rng(400 + k)
+        b = rng.standard_normal(n) * (1 + 0.5j if np.iscomplexobj(trip[2]) else 1)
+        plan.append((key, [capi.ICCG, capi.MRTR_SGS, capi.MRTR_IC][(k // 3) % 3], b))
+    eng = serve.Engine(slots=3)
+    futs = [eng.submit(serve.Request(key=key, triplets=systems[key][1], size=systems[key][0], rhs=b, solver=sv,
+                                     accel=1.1, max_ite=3000)) for key, sv, b in plan]
+    resp = [f.result(timeout=120) for f in futs]
+    eng.close()
+    print(f"element-data serving: {sum(r.cache_hit for r in resp)} cache hits of {len(resp)}, "
+          f"peak resident {eng.peak_resident} B", flush=True)
+    csr = {key: O.assemble(n, *trip, O.ASSEMBLE_BUILDER) for key, (n, trip) in systems.items()}
+    for (key, sv, b), r in zip(plan, resp):
+        assert r.error is None, r.error
+        rp, ci, v = csr[key]
+        m = capi.Matrix(ctx, rp, ci, v)
+        d_b = torch_cuda.from_numpy(np.ascontiguousarray(b)).cuda()
+        d_x = torch_cuda.zeros_like(d_b)
+        torch_cuda.cuda.synchronize()
+        rs = m.solve_device(sv, d_b.data_ptr(), d_x.data_ptr(), conv_cri=1e-8, max_ite=3000,
+                            normB=float(np.linalg.norm(b)), opts=capi.default_options(accel_factor=1.1))
+        ctx.synchronize()
+        m.close()
+        assert r.result.iterations == rs.iterations, (key, sv)
+        assert np.array_equal(r.x, d_x.cpu().numpy()), (key, sv)
