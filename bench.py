@@ -681,6 +681,85 @@ def workload_voxelfem(a):
     ctx.close()
 
 
+def workload_assemble(a):
+    """Element data to an analysed matrix on the GPU at the reference
+    example's size: the graded 95^3 mesh's edge connectivity with its
+    n x A = 0 edges eliminated, one 12 x 12 element matrix per element
+    (synthetic values: the assembly's work does not depend on them), triplets
+    in the example's add order (VoxelMatrix.cpp:130-198).  Timed: device
+    triplets -> assembled and analysed matrix (ssgpu_matrix_create_triplets),
+    beside the same CSR from host arrays (ssgpu_matrix_create: upload and
+    analysis only).  CPU: the compiled reference's SparseBuilder::add +
+    SparseMat(builder) on the first elements' triplets, one thread (the
+    builder is sequential), scaled to all triplets."""
+    import torch
+    from sparsesolv_amd import assembly, capi, voxel
+    t0 = time.perf_counter()
+    ax = voxel.voxelfem_axis()
+    mesh = voxel.tensor_mesh(ax, ax, ax)
+    nu, zero, t_edge = voxel.voxelfem_setup(mesh)
+    conn = np.where(zero, -1, np.cumsum(~zero) - 1)[mesh.elem_edges]
+    n = int((~zero).sum())
+    ke = np.random.default_rng(0).standard_normal((len(conn), 12, 12))
+    r, c, v = assembly.element_triplets(conn, ke)
+    del ke
+    nt = len(r)
+    print(f"# assembly input: {len(conn)} elements, N={n}, {nt} triplets ({time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    ctx = capi.Context(0)
+    dr, dc, dv = (torch.from_numpy(x).cuda() for x in (r, c, v))
+    torch.cuda.synchronize()
+    out = {"metric": "element data to analysed matrix, ms", "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "f64", "data": "the example's graded 95^3 mesh connectivity, synthetic element matrices",
+           "config": {"workload": f"VoxelFEM assembly, {len(conn)} elements, N={n}, {nt} triplets"}}
+    lat, lat_csr = [], []
+    m = None
+    for k in range(max(1, a.steps) + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        m = capi.Matrix.from_triplets(ctx, n, dr.data_ptr(), dc.data_ptr(), dv.data_ptr(),
+                                      mode=capi.ASSEMBLE_BUILDER, ntrip=nt, complex_=False)
+        if k:
+            lat.append(time.perf_counter() - t)
+        if k < max(1, a.steps):
+            m.close()
+    rp, ci, vals = m.csr()
+    m.close()
+    for k in range(max(1, a.steps) + 1):
+        t = time.perf_counter()
+        m2 = capi.Matrix(ctx, rp, ci, vals)
+        if k:
+            lat_csr.append(time.perf_counter() - t)
+        m2.close()
+    host = voxel.assemble_mesh(mesh, nu, t_edge=t_edge, zero_edge=zero)
+    same_pattern = bool(np.array_equal(rp, host.row_ptr) and np.array_equal(ci, host.col_idx))
+    out["value"] = 1e3 * statistics.median(lat)
+    out["gpu"] = {"triplets_to_matrix_ms": 1e3 * statistics.median(lat),
+                  "csr_to_matrix_ms": 1e3 * statistics.median(lat_csr),
+                  "nnz": int(len(ci)), "pattern_equals_host_assembly": same_pattern}
+    print(f"# triplets -> matrix {out['gpu']['triplets_to_matrix_ms']:.0f} ms, CSR -> matrix "
+          f"{out['gpu']['csr_to_matrix_ms']:.0f} ms, pattern equal {same_pattern}", file=sys.stderr, flush=True)
+    if not a.no_cpu:
+        try:
+            from oracle import oracle as O
+            if O.reference_available():
+                import ctypes
+                gomp = ctypes.CDLL("libgomp.so.1")
+                gomp.omp_set_num_threads(1)
+                ns = min(nt, 8_000_000)
+                t = time.perf_counter()
+                O.ref_assemble(n, r[:ns], c[:ns], v[:ns], O.ASSEMBLE_BUILDER)
+                tc = time.perf_counter() - t
+                out["cpu_baseline"] = {"value": 1e3 * tc * nt / ns, "unit": "ms", "cores": 1, "kind": "reference",
+                                       "sample": f"SparseBuilder + SparseMat(builder) on the first {ns} triplets: "
+                                                 f"{tc:.2f} s, scaled to {nt}"}
+                print(f"# cpu reference builder: {tc:.2f} s for {ns} triplets", file=sys.stderr, flush=True)
+        except OSError:
+            pass
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -702,7 +781,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=16, help="concurrent slots of the served run (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--workload", default="dropin",
-                    choices=["dropin", "batched", "serving", "eddy", "voxelfem", "block"],
+                    choices=["dropin", "batched", "serving", "eddy", "voxelfem", "block", "assemble"],
                     help="dropin: the n=95 solve (default); batched: 256 systems n=8..40 plus one n=60 system with "
                          "32 right-hand sides; serving: Poisson arrivals of mixed sizes / types")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -734,6 +813,8 @@ def main():
         return workload_block(a)
     if a.workload == "voxelfem":
         return workload_voxelfem(a)
+    if a.workload == "assemble":
+        return workload_assemble(a)
     t0 = time.perf_counter()
     s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
     t_gen = time.perf_counter() - t0
