@@ -166,6 +166,14 @@ int ssgpu_matrix_update_values(ssgpu_matrix m, const void* values);
 int ssgpu_matrix_destroy(ssgpu_matrix m);
 int ssgpu_matrix_info(ssgpu_matrix m, int32_t* n, int64_t* nnz, int* dtype,
                       int32_t* fwd_levels, int32_t* bwd_levels);
+/* The IC sweeps' launch shape, as chosen by the tuner at the first
+ * factorization (or fixed by SSGPU_SWEEP_MODE): *mode 0 sync-free 64-row
+ * chunks, 1 one launch per level, 2 tiles (level bands cut into row-index
+ * intervals, in-tile inputs handed over in LDS); *grid its persistent
+ * workgroups; *tiles the forward sweep's tile count (0 when none were
+ * built).  Every shape gives the same bits.  No counterpart in the reference
+ * (its sweeps are sequential loops, MatSolverBase.cpp:177-231). */
+int ssgpu_matrix_sweep_info(ssgpu_matrix m, int* mode, int* grid, int32_t* tiles);
 
 /* Solve A x = b like MatSolver*::solve(conv_cri, max_ite, normB, A, b, x)
  * (MatSolverBase.hpp:144-150): the factor is rebuilt on every call, as the

@@ -76,6 +76,7 @@ def lib():
         L.ssgpu_abmc_order.argtypes = [ctypes.c_int32, V, V, I, V, P(ctypes.c_int32)]
         L.ssgpu_matrix_create_triplets.argtypes = [V, I, ctypes.c_int32, ctypes.c_int64, V, V, V, I, I, P(V)]
         L.ssgpu_matrix_get_csr.argtypes = [V, V, V, V]
+        L.ssgpu_matrix_sweep_info.argtypes = [V, P(I), P(I), P(ctypes.c_int32)]
         L.ssgpu_matrix_update_values.argtypes = [V, V]
         L.ssgpu_matrix_destroy.argtypes = [V]
         L.ssgpu_matrix_info.argtypes = [V, P(ctypes.c_int32), P(ctypes.c_int64), P(I), P(ctypes.c_int32),
@@ -234,6 +235,12 @@ class Matrix:
                                                      ctypes.byref(self.h)))
         self.row_ptr, self.col_idx, _ = self.csr(values=False)
         return self
+
+    def sweep_info(self):
+        """(mode, grid, tiles): the IC sweeps' shape (0 chunks, 1 per level, 2 tiles), its workgroups, tile count."""
+        md, g, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int32()
+        check(lib().ssgpu_matrix_sweep_info(self.h, ctypes.byref(md), ctypes.byref(g), ctypes.byref(t)))
+        return md.value, g.value, t.value
 
     def csr(self, values=True):
         """(row_ptr, col_idx, values) as the library stores the matrix (values None when not asked)."""

@@ -20,9 +20,11 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "ssgpu_host.h"
+#include "ssgpu_tiles.h"
 
 namespace ssgpu {
 
@@ -204,6 +206,7 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
         if (!std::strcmp(e, "level")) out.by_level = true;
         if (!std::strcmp(e, "sf")) out.by_level = false;
     }
+    out.h_soff = soff;
     upload(c, out.slice_off, soff);
     upload(c, out.rlen, rlen);
     upload(c, out.ecol, ecol);
@@ -211,6 +214,35 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
     upload(c, out.dmap_f, df);
     upload(c, out.emap_a, ea);
     upload(c, out.dmap_a, da);
+}
+
+// tiles of a sweep on the device (the layout is made by tile_layout)
+// mean rows per level up to which tiles are built (see analyse)
+constexpr int64_t kTileRowsPerLevel = 64;
+
+static void build_tiles(Context* c, int n, const std::vector<int>& order, const SfRows& rows, const LevelCsr& lv,
+                        const std::vector<int>& soff, bool forward, int band, size_t lds_cap, size_t vsize,
+                        TileCsr& out) {
+    const TileLayout t = tile_layout(n, order, rows.off, rows.col, lv.levptr, forward, band, lds_cap, vsize);
+    out.ntiles = (int)t.tile_rp.size() - 1;
+    // each packed entry's place in the sliced-ELL arrays (entry k of position p
+    // at soff[p / 64] + p % 64 + 64 k), so a tile's entries stage in flat batches
+    std::vector<int> teidx(std::max<size_t>(t.tslot.size(), 1), 0);
+    parallel_for((int)t.tpos.size(), [&](int q) {
+        const int p = t.tpos[q];
+        const int base = soff[p / kSfRows] + p % kSfRows;
+        for (int e = t.tent[q]; e < t.tent[q + 1]; ++e) teidx[e] = base + kSfRows * (e - t.tent[q]);
+    });
+    upload(c, out.teidx, teidx);
+    for (int k = 0; k < 4; ++k) out.lds[k] = t.lds[k];
+    upload(c, out.tile_rp, t.tile_rp);
+    upload(c, out.tile_sp, t.tile_sp);
+    upload(c, out.step_rp, t.step_rp);
+    upload(c, out.tile_xp, t.tile_xp);
+    upload(c, out.ext_rows, t.ext_rows);
+    upload(c, out.tpos, t.tpos);
+    upload(c, out.tent, t.tent);
+    upload(c, out.tslot, t.tslot);
 }
 
 // upper(A) (SprsTrans.cpp:387-394: entries col >= row, diagonal first) in
@@ -363,6 +395,26 @@ static void analyse(Matrix& m) {
         }
         build_sf(c, n, order_l, rl, m.L, m.sfL);
         build_sf(c, n, order_u, ru, m.Uic, m.sfUic);
+        // Tiles for the tiled sweep when levels are narrow (at most
+        // kTileRowsPerLevel rows on average: chains, strips), where a 64-row
+        // chunk of the sync-free sweep holds rows of several levels and pays a
+        // global round trip per level; there tiles measured 15-21% faster
+        // (a 200,000-row tridiagonal chain, an 8 x 50,000 grid).  On wide
+        // levels (3-D grids) they lose, and are not built.  The tuner decides
+        // whether they run.  SSGPU_TILE=1 builds them always, 0 never.
+        const char* te = std::getenv("SSGPU_TILE");
+        int band = 64;
+        size_t cap = 64 << 10;
+        if (const char* e = std::getenv("SSGPU_TILE_BAND")) band = std::max(1, std::min(kTileMaxSteps, std::atoi(e)));
+        if (const char* e = std::getenv("SSGPU_TILE_LDS")) cap = (size_t)std::max(4096, std::atoi(e));
+        const bool narrow = (int64_t)n <= kTileRowsPerLevel * (int64_t)std::max(1, m.L.nlev);
+        const bool want = te ? std::strcmp(te, "0") != 0 : narrow;
+        if (want && m.sfL.kmax > 0 && m.sfUic.kmax > 0 && m.L.nlev >= 2) {
+            m.sfL.tile.reset(new TileCsr());
+            m.sfUic.tile.reset(new TileCsr());
+            build_tiles(c, n, order_l, rl, m.L, m.sfL.h_soff, true, band, cap, m.vsize, *m.sfL.tile);
+            build_tiles(c, n, order_u, ru, m.Uic, m.sfUic.h_soff, false, band, cap, m.vsize, *m.sfUic.tile);
+        }
     }
     // sync-free grid: 128 waves keep ~10^3-row levels fed with the least
     // polling; levels of thousands of rows want more waves in flight
@@ -491,6 +543,13 @@ struct Solver {
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
                   const T* aux, const T* dd, T* x) {
         (void)reverse;   // backward structures are already in their own level order
+        if (S.use_tile && S.tile && !S.by_level) {
+            const int pair = (sizeof(T) == 16 ? 2 : 0) + (sizeof(F) < sizeof(T) ? 1 : 0);
+            launch_sweep_tile<T, F>(s, mode, S.kmax, m.tile_grid, S.tile->lds[pair], S.tile->args(), ticket,
+                                    errflag(), spin_limit, S.srow, S.slice_off.as<int>(), S.rlen.as<int>(), ev, dv,
+                                    rhs, aux, dd, m.sweep_idle, x);
+            return;
+        }
         if (S.by_level)
             launch_sweep_levels<T, F>(s, mode, S.kmax, S.lv->nlev, S.lv->levptr.data(), S.srow,
                                       S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux,
@@ -656,32 +715,58 @@ struct Solver {
     // IC sweeps of this matrix.  Every shape gives the same bits.
     template <typename T> void tune_sweeps() {
         m.sweeps_tuned = true;
-        const bool fix_mode = std::getenv("SSGPU_SWEEP_MODE") != nullptr;
+        const char* mode_env = std::getenv("SSGPU_SWEEP_MODE");
+        const bool fix_mode = mode_env != nullptr;
         const bool fix_grid = std::getenv("SSGPU_SWEEP_GRID") != nullptr;
-        if (fix_mode && fix_grid) return;
+        const bool want_tile = fix_mode && !std::strcmp(mode_env, "tile");
+        // a tiled shape exists when both directions have tiles whose shared
+        // memory fits a workgroup (gfx950: 160 KB)
+        const int pair = sizeof(T) == 16 ? 2 : 0;
+        const bool have_tile = m.sfL.tile && m.sfUic.tile && m.sfL.tile->ntiles > 0 && m.sfUic.tile->ntiles > 0 &&
+                               std::max(m.sfL.tile->lds[pair], m.sfUic.tile->lds[pair]) <= (160u << 10);
+        struct Shape { int kind; int grid; };   // 0 sync-free chunks, 1 one launch per level, 2 tiles
+        auto apply = [&](const Shape& sh) {
+            m.sfL.by_level = m.sfUic.by_level = sh.kind == 1;
+            m.sfL.use_tile = m.sfUic.use_tile = sh.kind == 2;
+            if (sh.kind == 0) m.sweep_grid = sh.grid;
+            if (sh.kind == 2) m.tile_grid = sh.grid;
+        };
+        std::vector<Shape> cand;
+        if (want_tile) {
+            if (!have_tile) return;
+            if (const char* g = std::getenv("SSGPU_TILE_GRID")) {
+                apply({2, std::max(1, std::atoi(g))});
+                return;
+            }
+            for (int g : {256, 512}) cand.push_back({2, g});
+        } else {
+            if (fix_mode && fix_grid) return;
+            if (!fix_mode || !m.sfL.by_level) {
+                for (int g : {128, 160, 1024})
+                    if (!fix_grid || g == m.sweep_grid) cand.push_back({0, fix_grid ? m.sweep_grid : g});
+            }
+            if ((!fix_mode || m.sfL.by_level) && m.L.nlev <= kTuneMaxLevels) cand.push_back({1, m.sweep_grid});
+            if (!fix_mode && have_tile)
+                for (int g : {256, 512}) cand.push_back({2, g});
+        }
+        if (cand.size() < 2) {
+            if (cand.size() == 1 && want_tile) apply(cand[0]);
+            return;
+        }
         DevBuf r, out;
         r.alloc(c, sizeof(T) * m.n);
         out.alloc(c, sizeof(T) * m.n);
         launch_fill<T>(s, m.n, r.as<T>(), one<T>());
-        struct Shape { bool level; int grid; };
-        std::vector<Shape> cand;
-        if (!fix_mode || !m.sfL.by_level) {
-            for (int g : {128, 160, 1024})
-                if (!fix_grid || g == m.sweep_grid) cand.push_back({false, fix_grid ? m.sweep_grid : g});
-        }
-        if ((!fix_mode || m.sfL.by_level) && m.L.nlev <= kTuneMaxLevels) cand.push_back({true, m.sweep_grid});
-        if (cand.size() < 2) return;
         hipEvent_t e0, e1;
         SSGPU_HIP(hipEventCreate(&e0));
         SSGPU_HIP(hipEventCreate(&e1));
         float best = 1e30f;
         Shape pick = cand[0];
         for (const Shape& sh : cand) {
-            m.sfL.by_level = sh.level;
-            m.sweep_grid = sh.grid;
+            apply(sh);
             float t = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
-                if (!sh.level) launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
+                if (sh.kind != 1) launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
                 SSGPU_HIP(hipEventRecord(e0, s));
                 sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(),
                                r.as<T>(), nullptr, nullptr, out.as<T>());
@@ -696,8 +781,7 @@ struct Solver {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         check_sweep_error();
-        m.sfL.by_level = m.sfUic.by_level = pick.level;
-        m.sweep_grid = pick.grid;
+        apply(pick);
     }
     template <typename T> int factor(const T* av, double accel) {
         const int64_t nl = m.L.nnz;
@@ -1993,6 +2077,16 @@ int ssgpu_matrix_info(ssgpu_matrix mh, int32_t* n, int64_t* nnz, int* dtype, int
     if (dtype) *dtype = mh->m.dtype;
     if (fwd_levels) *fwd_levels = mh->m.L.nlev;
     if (bwd_levels) *bwd_levels = mh->m.Uic.nlev;
+    return SSGPU_OK;
+}
+
+int ssgpu_matrix_sweep_info(ssgpu_matrix mh, int* mode, int* grid, int32_t* tiles) {
+    if (!mh) return SSGPU_BAD_ARG;
+    const Matrix& m = mh->m;
+    const int k = m.sfL.use_tile ? 2 : m.sfL.by_level ? 1 : 0;
+    if (mode) *mode = k;
+    if (grid) *grid = k == 2 ? m.tile_grid : m.sweep_grid;
+    if (tiles) *tiles = m.sfL.tile ? m.sfL.tile->ntiles : 0;
     return SSGPU_OK;
 }
 

@@ -110,7 +110,23 @@ struct LevelCsr {
 // Off-diagonal entries only, in summation order; the diagonal is kept per
 // position.  Two value sources: the level-ordered factor (f) and the matrix
 // values (a).
+// Tiles of a sweep (k_sweep_tile): device arrays of TileArgs and the largest
+// tile's shared memory for each value-type pair (FP64/FP64, FP64/FP32,
+// complex/complex, complex/complex64).
+struct TileCsr {
+    int ntiles = 0;
+    size_t lds[4] = {0, 0, 0, 0};
+    DevBuf tile_rp, tile_sp, step_rp, tile_xp, ext_rows, tpos, tent, tslot, teidx;
+    TileArgs args() const {
+        return TileArgs{ntiles, tile_rp.as<int>(), tile_sp.as<int>(), step_rp.as<int>(), tile_xp.as<int>(),
+                        ext_rows.as<int>(), tpos.as<int>(), tent.as<int>(), tslot.as<unsigned short>(),
+                        teidx.as<int>()};
+    }
+};
+
 struct SfCsr {
+    std::unique_ptr<TileCsr> tile;   // tiles of this sweep (built for long level chains), or null
+    bool use_tile = false;           // sweeps through the tiles (chosen by the tuner)
     int kmax = 0;          // longest row; 0 when a row exceeds 64 entries (level path used)
     int64_t nell = 0;
     const int* srow = nullptr;   // level position -> row (the LevelCsr's prow)
@@ -118,6 +134,7 @@ struct SfCsr {
     bool by_level = false;       // one launch per level instead of the sync-free sweep
     DevBuf slice_off, rlen, ecol, emap_f, dmap_f, emap_a, dmap_a;
     DevBuf eval, dval, eval32, dval32;
+    std::vector<int> h_soff;     // slice offsets (host copy, for the tile layout's entry indices)
 };
 
 struct Matrix {
@@ -141,6 +158,7 @@ struct Matrix {
     int ic_max_updates = 0;    // largest update list of a row
     DevBuf ic_po, ic_slot, ic_pos, ic_col;
     int sweep_grid = 128;     // persistent one-wave workgroups per sweep
+    int tile_grid = 256;      // persistent one-wave workgroups of a tiled sweep
     bool sweeps_tuned = false;   // sweep launch shape measured (first factorization)
     int spmv_stage_kb = 16;      // SpMV staging (16 or 32 KB), measured at creation
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)

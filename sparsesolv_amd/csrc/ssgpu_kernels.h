@@ -8,6 +8,7 @@
 #pragma once
 
 #include "ssgpu_common.h"
+#include "ssgpu_tiles.h"
 
 #include <tuple>
 #include <utility>
@@ -144,6 +145,29 @@ void launch_sweep_sf(hipStream_t s, int mode, int kmax, int n, int grid, int* ti
                      const int* srow,
                      const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                      const T* rhs, const T* aux, const T* dd, int max_idle, T* x);
+// Tiled sweep: the rows of a band of consecutive levels, cut into intervals of
+// row index, form a tile (layout: ssgpu_tiles.h); a one-wave workgroup takes
+// tiles in ticket order, waits for the tile's external inputs, stages its
+// entries in LDS and walks the tile's levels ("steps") with every in-tile
+// input handed over through LDS.  Packed rows are tile by tile, step by step.
+struct TileArgs {
+    int ntiles;
+    const int* tile_rp;    // ntiles + 1: packed rows of each tile
+    const int* tile_sp;    // ntiles + 1: steps of each tile in step_rp
+    const int* step_rp;    // first packed row of each step
+    const int* tile_xp;    // ntiles + 1: external inputs of each tile in ext_rows
+    const int* ext_rows;   // rows a tile reads from earlier tiles
+    const int* tpos;       // packed row -> its level-order position
+    const int* tent;       // packed row -> first packed entry (packed rows + 1 entries)
+    const unsigned short* tslot;   // packed entry -> LDS slot of its input (in-tile rows, then externals)
+    const int* teidx;      // packed entry -> its index in the sweep's sliced-ELL value arrays
+};
+// one tiled sweep; lds_bytes = the largest tile's shared-memory footprint
+template <typename T, typename F>
+void launch_sweep_tile(hipStream_t s, int mode, int kmax, int grid, size_t lds_bytes, const TileArgs& ta,
+                       int* ticket, int* err, long spin_limit, const int* srow, const int* slice_off,
+                       const int* rlen, const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd,
+                       int max_idle, T* x);
 // the same sweep one level per launch (levptr: host array of nlev + 1 level
 // starts in the level order); for orderings with few, wide levels
 template <typename T, typename F>

@@ -248,6 +248,191 @@ __global__ __launch_bounds__(kSfRows) void k_batch_sweep(int norder, const int2*
     }
 }
 
+// Tiled sweep.  A tile is the rows of a band of consecutive levels within an
+// interval of row index; a row's inputs lie at lower levels and, inside its
+// band, on the near side of its interval, so they come from the same tile or
+// from tiles with smaller tickets (deadlock-free like the chunked sweep).
+// One wave per tile:
+//   1. it loads the tile's external inputs (agent-scope polls, repeated until
+//      none is the sentinel) and meanwhile stages the tile's rows (row,
+//      right-hand side, diagonal) and entries (value, LDS slot of the input)
+//      in LDS;
+//   2. it walks the tile's levels: a lane per row, inputs read from LDS, the
+//      result written to LDS for the rows after it and published with an
+//      agent-scope store for later tiles.
+// Inside a tile a level costs an LDS round trip instead of a global one.
+// Each row is formed by the same code, in the same order, as in the other
+// sweeps (RowState::result).
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(kTileThreads) void k_sweep_tile(TileArgs ta, int* __restrict__ ticket,
+                                                             int* __restrict__ err, long spin_limit,
+                                                             const int* __restrict__ srow,
+                                                             const int* __restrict__ slice_off,
+                                                             const int* __restrict__ rlen,
+                                                             const F* __restrict__ eval, const F* __restrict__ dval,
+                                                             const T* __restrict__ rhs, const T* __restrict__ aux,
+                                                             const T* __restrict__ dd, int max_idle,
+                                                             T* __restrict__ x) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    constexpr int NT = kTileThreads;
+    constexpr int KX = kTileExtPerLane;
+    long spins = 0;
+    while (true) {
+        const int t = take_ticket(ticket);
+        if (t >= ta.ntiles) return;
+        const int r0 = ta.tile_rp[t], R = ta.tile_rp[t + 1] - r0;
+        const int x0 = ta.tile_xp[t], X = ta.tile_xp[t + 1] - x0;
+        const int e0 = ta.tent[r0], E = ta.tent[r0 + R] - e0;
+        const int s0 = ta.tile_sp[t], ns = ta.tile_sp[t + 1] - s0;
+        T* sx = reinterpret_cast<T*>(smem);   // R in-tile rows, then X externals
+        T* sb = sx + (R + X);                 // right-hand side (aux for BWD_IC)
+        T* sd = sb + R;                       // diagonal (dd for BWD_IC)
+        F* sv = reinterpret_cast<F*>(sd + R);
+        int* srw = reinterpret_cast<int*>(sv + E);
+        int* sof = srw + R;
+        unsigned short* ss = reinterpret_cast<unsigned short*>(sof + R + 1);
+        // the external inputs' rows (a slot past X reads row 0, never stored)
+        int xr[KX];
+#pragma unroll
+        for (int b = 0; b < KX; ++b) {
+            const int k = lane + NT * b;
+            xr[b] = k < X ? ta.ext_rows[x0 + k] : -1;
+        }
+        // first packed row of each step, one per lane
+        const int step_q = lane < ns ? ta.step_rp[s0 + lane] - r0 : R;
+        // Stage rows and entries in batches: every load of a batch goes out
+        // before the first is waited for (a load per loop trip would cost a
+        // memory round trip each).  Slots past the end read entry / row 0.
+        constexpr int JR = 8;
+        for (int q0 = 0; q0 < R; q0 += NT * JR) {
+            int pp[JR], off[JR], ii[JR];
+#pragma unroll
+            for (int j = 0; j < JR; ++j) {
+                const int q = q0 + lane + NT * j;
+                const int qq = q < R ? q : 0;
+                pp[j] = ta.tpos[r0 + qq];
+                off[j] = ta.tent[r0 + qq] - e0;
+            }
+            F dg[JR];
+#pragma unroll
+            for (int j = 0; j < JR; ++j) {
+                ii[j] = srow[pp[j]];
+                dg[j] = MODE == BWD_IC ? F() : dval[pp[j]];
+            }
+            T bv[JR], dv[JR];
+#pragma unroll
+            for (int j = 0; j < JR; ++j) {
+                bv[j] = MODE == BWD_IC ? aux[ii[j]] : rhs[ii[j]];
+                dv[j] = MODE == BWD_IC ? dd[ii[j]] : wid<T, F>(dg[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < JR; ++j) {
+                const int q = q0 + lane + NT * j;
+                if (q < R) {
+                    srw[q] = ii[j];
+                    sof[q] = off[j];
+                    sb[q] = bv[j];
+                    sd[q] = dv[j];
+                }
+            }
+        }
+        constexpr int JE = 32;
+        for (int b0 = 0; b0 < E; b0 += NT * JE) {
+            int ei[JE];
+            unsigned short sl[JE];
+#pragma unroll
+            for (int j = 0; j < JE; ++j) {
+                const int e = b0 + lane + NT * j;
+                const int ee = e < E ? e : 0;
+                ei[j] = ta.teidx[e0 + ee];
+                sl[j] = ta.tslot[e0 + ee];
+            }
+            F ev[JE];
+#pragma unroll
+            for (int j = 0; j < JE; ++j) ev[j] = eval[ei[j]];
+#pragma unroll
+            for (int j = 0; j < JE; ++j) {
+                const int e = b0 + lane + NT * j;
+                if (e < E) {
+                    sv[e] = ev[j];
+                    ss[e] = sl[j];
+                }
+            }
+        }
+        if (lane == 0) sof[R] = E;
+        // poll the external inputs until every one is present
+        unsigned miss = 0;
+#pragma unroll
+        for (int b = 0; b < KX; ++b)
+            if (xr[b] >= 0) miss |= 1u << b;
+        int idle = 0;
+        bool ok = true;
+        while (true) {
+            __asm__ __volatile__("" ::: "memory");
+            const unsigned want = miss;
+            T v[KX];
+#pragma unroll
+            for (int b = 0; b < KX; ++b) v[b] = gload(x + (xr[b] >= 0 ? xr[b] : 0));
+#pragma unroll
+            for (int b = 0; b < KX; ++b)
+                if (((want >> b) & 1u) && !sent(v[b])) {
+                    sx[R + lane + NT * b] = v[b];
+                    miss &= ~(1u << b);
+                }
+            if (__all(miss == 0u)) break;
+            const bool any_progress = __any(miss != want);
+            if (any_progress && spin_limit > 0) spins = 0;
+            if (++spins > spin_limit) {
+                atomicOr(err, 1);
+                ok = false;
+                break;
+            }
+            if (any_progress) {
+                idle = 0;
+            } else {
+                if (idle < max_idle) ++idle;
+                if (idle <= 1) __builtin_amdgcn_s_sleep(1);
+                else if (idle == 2) __builtin_amdgcn_s_sleep(2);
+                else if (idle == 3) __builtin_amdgcn_s_sleep(4);
+                else if (idle == 4) __builtin_amdgcn_s_sleep(8);
+                else __builtin_amdgcn_s_sleep(16);
+            }
+        }
+        if (!ok) return;
+        __syncthreads();   // staging and externals in LDS
+        for (int st = 0; st < ns; ++st) {
+            const int qa = __shfl(step_q, st);
+            const int qb = st + 1 < ns ? __shfl(step_q, st + 1) : R;
+            for (int q = qa + lane; q < qb; q += NT) {
+                const int ka = sof[q], len = sof[q + 1] - ka;
+                F vals[KMAX];
+                unsigned short sl[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    sl[k] = k < len ? ss[ka + k] : (unsigned short)0;
+                    vals[k] = k < len ? sv[ka + k] : F();
+                }
+                T g[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) g[k] = sx[sl[k]];
+                T acc = (MODE == BWD_IC) ? Num<T>::zero() : sb[q];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (k < len) acc = Num<T>::sub(acc, Num<T>::mul(wid<T, F>(vals[k]), g[k]));
+                const T res = (MODE == BWD_IC) ? Num<T>::add(Num<T>::mul(acc, sd[q]), sb[q]) : Num<T>::div(acc, sd[q]);
+                sx[q] = res;
+                gstore(x + srw[q], res);
+            }
+            // the next level reads this one's results from LDS: wait for the
+            // LDS writes only (lgkmcnt 0; vmcnt and expcnt left at their
+            // maxima), not for the global stores, which only later tiles read
+            __builtin_amdgcn_s_waitcnt(0xC07F);
+            __builtin_amdgcn_s_barrier();
+        }
+    }
+}
+
 // One level of a sweep, for orderings whose levels are wide (few levels of
 // many rows each, as a multi-color order has): every input of a row of level
 // l was written by an earlier launch on the same stream, so plain loads see
@@ -595,6 +780,47 @@ void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder,
                                               out, max_idle, rec);
 }
 
+template <typename T, typename F, int MODE>
+void sweep_tile_kmax(hipStream_t s, int kmax, int g, size_t lds, const TileArgs& ta, int* ticket, int* err,
+                     long spin_limit, const int* srow, const int* slice_off, const int* rlen, const F* eval,
+                     const F* dval, const T* rhs, const T* aux, const T* dd, int max_idle, T* x) {
+#define SSGPU_TL(K)                                                                                             \
+    do {                                                                                                        \
+        auto fn = k_sweep_tile<T, F, MODE, K>;                                                                  \
+        if (lds > 65536)                                                                                        \
+            SSGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),                                    \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));              \
+        hipLaunchKernelGGL(fn, dim3(g), dim3(kTileThreads), lds, s, ta, ticket, err, spin_limit, srow,          \
+                           slice_off, rlen, eval, dval, rhs, aux, dd, max_idle, x);                             \
+    } while (0)
+    if (kmax <= 8) SSGPU_TL(8);
+    else if (kmax <= 12) SSGPU_TL(12);
+    else if (kmax <= 16) SSGPU_TL(16);
+    else if (kmax <= 20) SSGPU_TL(20);
+    else if (kmax <= 24) SSGPU_TL(24);
+    else if (kmax <= 32) SSGPU_TL(32);
+    else SSGPU_TL(64);
+#undef SSGPU_TL
+}
+
+template <typename T, typename F>
+void launch_sweep_tile(hipStream_t s, int mode, int kmax, int grid, size_t lds_bytes, const TileArgs& ta,
+                       int* ticket, int* err, long spin_limit, const int* srow, const int* slice_off,
+                       const int* rlen, const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd,
+                       int max_idle, T* x) {
+    if (ta.ntiles <= 0) return;
+    const int g = std::max(1, std::min(grid, ta.ntiles));
+    if (mode == FWD_DIV_LAST)
+        sweep_tile_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, lds_bytes, ta, ticket, err, spin_limit, srow, slice_off, rlen,
+                                           eval, dval, rhs, aux, dd, max_idle, x);
+    else if (mode == BWD_IC)
+        sweep_tile_kmax<T, F, BWD_IC>(s, kmax, g, lds_bytes, ta, ticket, err, spin_limit, srow, slice_off, rlen,
+                                     eval, dval, rhs, aux, dd, max_idle, x);
+    else
+        sweep_tile_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, lds_bytes, ta, ticket, err, spin_limit, srow, slice_off,
+                                            rlen, eval, dval, rhs, aux, dd, max_idle, x);
+}
+
 template <typename T>
 void launch_sf_prepare(hipStream_t s, int n, T* a, T* b, int* tickets) {
     hipLaunchKernelGGL(k_sf_prepare<T>, dim3(grid_for(n)), dim3(kBS), 0, s, n, a, b, tickets);
@@ -643,6 +869,14 @@ template void launch_sweep_levels<zc, zc>(hipStream_t, int, int, int, const int*
 template void launch_sweep_levels<zc, fc>(hipStream_t, int, int, int, const int*, const int*, const int*,
                                              const int*, const int*, const fc*, const fc*, const zc*, const zc*,
                                              const zc*, zc*);
+#define SSGPU_TLI(T, F)                                                                                      \
+    template void launch_sweep_tile<T, F>(hipStream_t, int, int, int, size_t, const TileArgs&, int*, int*, long, \
+                                          const int*, const int*, const int*, const F*, const F*, const T*,     \
+                                          const T*, const T*, int, T*);
+SSGPU_TLI(double, double)
+SSGPU_TLI(double, float)
+SSGPU_TLI(zc, zc)
+SSGPU_TLI(zc, fc)
 #define SSGPU_BSI(T, F)                                                                                   \
     template void launch_batch_sweep<T, F>(hipStream_t, int, int, int, int, const int2*, int*, int*, long,       \
                                            const BatchSys*, const BatchState*, int, int, int, int, GraphRec*);
