@@ -88,7 +88,8 @@ const double* read_scalars(Context* c) {
 
 // levels of the DAG "row i depends on the entries of its row with index on
 // the solved side"; forward = entries with col < i, backward = col > i.
-static std::vector<int> levels(int n, const std::vector<int>& rp, const std::vector<int>& ci, bool forward) {
+template <typename CI>
+static std::vector<int> levels(int n, const std::vector<int>& rp, const CI& ci, bool forward) {
     std::vector<int> lev(n, 0);
     if (forward) {
         for (int i = 0; i < n; ++i) {
@@ -109,9 +110,10 @@ static std::vector<int> levels(int n, const std::vector<int>& rp, const std::vec
 }
 
 // Build a LevelCsr from a host CSR (rp, ci) plus per-entry source map.
-static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const std::vector<int>& ci,
-                            const std::vector<int>& src, bool forward, LevelCsr& out,
-                            std::vector<int>* pos_out, std::vector<int>* order_out = nullptr) {
+template <typename CI, typename SRC>
+static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const CI& ci, const SRC& src,
+                            bool forward, LevelCsr& out, std::vector<int>* pos_out,
+                            std::vector<int>* order_out = nullptr) {
     const std::vector<int> lev = levels(n, rp, ci, forward);
     int nlev = 0;
     for (int i = 0; i < n; ++i) nlev = std::max(nlev, lev[i] + 1);
@@ -124,7 +126,8 @@ static void build_level_csr(Context* c, int n, const std::vector<int>& rp, const
         std::vector<int> fill(cnt.begin(), cnt.end() - 1);
         for (int i = 0; i < n; ++i) order[fill[lev[i]]++] = i;
     }
-    std::vector<int> prp(n + 1, 0), pci(rp[n]), prow(n), map(rp[n]);
+    std::vector<int> prp(n + 1, 0);
+    hvec<int> pci(rp[n]), prow(n), map(rp[n]);
     if (pos_out) pos_out->assign(n, 0);
     for (int p = 0; p < n; ++p) prp[p + 1] = prp[p] + (rp[order[p] + 1] - rp[order[p]]);
     parallel_for(n, [&](int p) {
@@ -156,19 +159,22 @@ constexpr int64_t kLevelSyncRows = 131072;
 constexpr int kTuneMaxLevels = 512;
 
 struct SfRows {
-    std::vector<int> off, col, sf, sa, df, da;
+    std::vector<int> off, df, da;
+    hvec<int> col, sf, sa;   // written in full by the caller after alloc()
     explicit SfRows(int n) : off(n + 1, 0), df(n, -1), da(n, -1) {}
-    void alloc() {
+    void alloc(bool with_sa = true) {   // with_sa false: no matrix-value sources (sa stays empty)
         for (size_t i = 0; i + 1 < off.size(); ++i) off[i + 1] += off[i];
         col.resize(off.back());
         sf.resize(off.back());
-        sa.resize(off.back());
+        if (with_sa) sa.resize(off.back());
     }
     int len(int i) const { return off[i + 1] - off[i]; }
 };
 
+// with_f / with_a: keep the value maps from the factor / from the matrix
+// (a structure filled from only one of them needs no map for the other)
 static void build_sf(Context* c, int n, const std::vector<int>& order, const SfRows& rows, const LevelCsr& lv,
-                     SfCsr& out) {
+                     SfCsr& out, bool with_f, bool with_a) {
     int kmax = 0;
     for (int i = 0; i < n; ++i) kmax = std::max(kmax, rows.len(i));
     const int nsl = (n + 63) / 64;
@@ -179,20 +185,35 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
         soff[sl + 1] = soff[sl] + 64 * w;
     }
     const int64_t nell = soff[nsl];
-    std::vector<int> ecol(std::max<int64_t>(nell, 1), 0), ef(std::max<int64_t>(nell, 1), -1),
-        ea(std::max<int64_t>(nell, 1), -1), df(n), da(n);
-    parallel_for(n, [&](int p) {
-        const int i = order[p];
-        const int a = rows.off[i], len = rows.len(i);
-        rlen[p] = len;
-        const int base = soff[p / 64] + (p % 64);
-        for (int k = 0; k < len; ++k) {
-            ecol[base + 64 * k] = rows.col[a + k];
-            ef[base + 64 * k] = rows.sf[a + k];
-            ea[base + 64 * k] = rows.sa[a + k];
+    // padding slots: column 0, no value source (-1); each slice is filled by
+    // the thread that writes its rows
+    hvec<int> ecol(std::max<int64_t>(nell, 1)), ef(with_f ? std::max<int64_t>(nell, 1) : 0),
+        ea(with_a ? std::max<int64_t>(nell, 1) : 0);
+    std::vector<int> df(n), da(n);
+    if (nell == 0) {
+        ecol[0] = 0;
+        if (with_f) ef[0] = -1;
+        if (with_a) ea[0] = -1;
+    }
+    parallel_for(nsl, [&](int sl) {
+        for (int64_t e = soff[sl]; e < soff[sl + 1]; ++e) {
+            ecol[e] = 0;
+            if (with_f) ef[e] = -1;
+            if (with_a) ea[e] = -1;
         }
-        df[p] = rows.df[i];
-        da[p] = rows.da[i];
+        for (int p = 64 * sl; p < std::min(n, 64 * sl + 64); ++p) {
+            const int i = order[p];
+            const int a = rows.off[i], len = rows.len(i);
+            rlen[p] = len;
+            const int base = soff[sl] + (p % 64);
+            for (int k = 0; k < len; ++k) {
+                ecol[base + 64 * k] = rows.col[a + k];
+                if (with_f) ef[base + 64 * k] = rows.sf[a + k];
+                if (with_a) ea[base + 64 * k] = rows.sa[a + k];
+            }
+            df[p] = rows.df[i];
+            da[p] = rows.da[i];
+        }
     });
     out.kmax = kmax <= 64 ? std::max(kmax, 1) : 0;
     out.nell = nell;
@@ -210,10 +231,14 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
     upload(c, out.slice_off, soff);
     upload(c, out.rlen, rlen);
     upload(c, out.ecol, ecol);
-    upload(c, out.emap_f, ef);
-    upload(c, out.dmap_f, df);
-    upload(c, out.emap_a, ea);
-    upload(c, out.dmap_a, da);
+    if (with_f) {
+        upload(c, out.emap_f, ef);
+        upload(c, out.dmap_f, df);
+    }
+    if (with_a) {
+        upload(c, out.emap_a, ea);
+        upload(c, out.dmap_a, da);
+    }
 }
 
 // tiles of a sweep on the device (the layout is made by tile_layout)
@@ -223,7 +248,7 @@ constexpr int64_t kTileRowsPerLevel = 64;
 static void build_tiles(Context* c, int n, const std::vector<int>& order, const SfRows& rows, const LevelCsr& lv,
                         const std::vector<int>& soff, bool forward, int band, size_t lds_cap, size_t vsize,
                         TileCsr& out) {
-    const TileLayout t = tile_layout(n, order, rows.off, rows.col, lv.levptr, forward, band, lds_cap, vsize);
+    const TileLayout t = tile_layout(n, order, rows.off, rows.col.data(), lv.levptr, forward, band, lds_cap, vsize);
     out.ntiles = (int)t.tile_rp.size() - 1;
     // each packed entry's place in the sliced-ELL arrays (entry k of position p
     // at soff[p / 64] + p % 64 + 64 k), so a tile's entries stage in flat batches
@@ -276,7 +301,7 @@ static void ensure_upper(Matrix& m) {
             ++q;
         }
     }
-    build_sf(c, n, order_s, rs, m.Us, m.sfUs);
+    build_sf(c, n, order_s, rs, m.Us, m.sfUs, false, true);   // SGS values come from S A S
     m.us_built = true;
 }
 
@@ -316,7 +341,7 @@ static void analyse(Matrix& m) {
         lrp[i + 1] = cnt;
     });
     for (int i = 0; i < n; ++i) lrp[i + 1] += lrp[i];
-    std::vector<int> lci(lrp[n]), lsrc(lrp[n]);
+    hvec<int> lci(lrp[n]), lsrc(lrp[n]);
     parallel_for(n, [&](int i) {
         int q = lrp[i];
         for (int k = rp[i]; k < rp[i + 1]; ++k)
@@ -336,7 +361,8 @@ static void analyse(Matrix& m) {
         std::vector<int> urp(n + 1, 0);
         for (size_t k = 0; k < lci.size(); ++k) urp[lci[k] + 1]++;
         for (int i = 0; i < n; ++i) urp[i + 1] += urp[i];
-        std::vector<int> uci(lci.size()), usrc(lci.size()), fill(urp.begin(), urp.end() - 1);
+        hvec<int> uci(lci.size()), usrc(lci.size());
+        std::vector<int> fill(urp.begin(), urp.end() - 1);
         std::vector<int> lstart(n);   // start of original row i inside level-ordered L
         {
             std::vector<int> prp_h(n + 1, 0);
@@ -359,14 +385,15 @@ static void analyse(Matrix& m) {
         m.uic_built = true;
         // sync-free copies: lower(A) (off-diagonals, diagonal = last entry) and transpose(lower)
         SfRows rl(n), ru(n);
-        for (int i = 0; i < n; ++i)
-            for (int k = lrp[i]; k < lrp[i + 1]; ++k)
-                if (lci[k] < i) {
-                    rl.off[i + 1]++;
-                    ru.off[lci[k] + 1]++;
-                }
+        // off-diagonal counts: row i of lower(A) and row i of its transpose,
+        // less the diagonal when the row has it (the last entry of a lower row)
+        parallel_for(n, [&](int i) {
+            const int dg = (lrp[i + 1] > lrp[i] && lci[lrp[i + 1] - 1] == i) ? 1 : 0;
+            rl.off[i + 1] = lrp[i + 1] - lrp[i] - dg;
+            ru.off[i + 1] = urp[i + 1] - urp[i] - dg;
+        });
         rl.alloc();
-        ru.alloc();
+        ru.alloc(false);   // the transpose is filled from the factor only
         parallel_for(n, [&](int i) {   // rows of lower(A): each row's own range
             const int a = lrp[i], e = lrp[i + 1];
             int ql = rl.off[i];
@@ -382,19 +409,20 @@ static void analyse(Matrix& m) {
                 }
             }
         });
-        std::vector<int> fu(ru.off.begin(), ru.off.end() - 1);
-        for (int i = 0; i < n; ++i) {   // ascending i: each transposed row fills in ascending order
-            const int a = lrp[i], e = lrp[i + 1];
-            for (int k = a; k < e; ++k) {
-                const int j = lci[k];
-                if (j < i) {
-                    const int qu = fu[j]++;
-                    ru.col[qu] = i; ru.sf[qu] = lstart[i] + (k - a); ru.sa[qu] = -1;
+        // row j of the transpose: the transposed structure's row j (rows i of
+        // lower(A) holding column j, ascending, each entry's place in the
+        // level-ordered factor) without its diagonal
+        parallel_for(n, [&](int j) {
+            int q = ru.off[j];
+            for (int e = urp[j]; e < urp[j + 1]; ++e)
+                if (uci[e] != j) {
+                    ru.col[q] = uci[e];
+                    ru.sf[q] = usrc[e];
+                    ++q;
                 }
-            }
-        }
-        build_sf(c, n, order_l, rl, m.L, m.sfL);
-        build_sf(c, n, order_u, ru, m.Uic, m.sfUic);
+        });
+        build_sf(c, n, order_l, rl, m.L, m.sfL, true, true);        // IC factor, or SGS lower(S A S)
+        build_sf(c, n, order_u, ru, m.Uic, m.sfUic, true, false);   // the factor's transpose only
         // Tiles for the tiled sweep when levels are narrow (at most
         // kTileRowsPerLevel rows on average: chains, strips), where a 64-row
         // chunk of the sync-free sweep holds rows of several levels and pays a

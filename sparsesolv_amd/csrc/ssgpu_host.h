@@ -67,7 +67,23 @@ DevBuf& scal_buf(Context* c);
 // error flag is cleared and reported
 const double* read_scalars(Context* c);
 
-template <typename T> void upload(Context* c, DevBuf& d, const std::vector<T>& h) {
+// Host arrays of the analysis that are written in full after allocation:
+// elements are default-initialized (not zeroed), so a large array costs no
+// single-threaded fill before the (parallel) loop that writes it.
+template <typename T> struct NoInit {
+    using value_type = T;
+    NoInit() = default;
+    template <typename U> NoInit(const NoInit<U>&) {}
+    T* allocate(size_t n) { return std::allocator<T>().allocate(n); }
+    void deallocate(T* p, size_t n) { std::allocator<T>().deallocate(p, n); }
+    template <typename U> void construct(U* p) noexcept { ::new ((void*)p) U; }
+    template <typename U, typename... A> void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+    template <typename U> bool operator==(const NoInit<U>&) const { return true; }
+    template <typename U> bool operator!=(const NoInit<U>&) const { return false; }
+};
+template <typename T> using hvec = std::vector<T, NoInit<T>>;
+
+template <typename T, typename A> void upload(Context* c, DevBuf& d, const std::vector<T, A>& h) {
     d.alloc(c, sizeof(T) * std::max<size_t>(h.size(), 1));
     // synchronous: h may be a temporary of the analysis
     if (!h.empty()) SSGPU_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));

@@ -52,9 +52,9 @@ struct TileLayout {
 // n rows; order: level-order position -> row; row i's off-diagonal inputs are
 // col[off[i] .. off[i+1]) in summation order; levptr: level starts in the
 // level order.  vsize: bytes of a value of the matrix (8 or 16).
-inline TileLayout tile_layout(int n, const std::vector<int>& order, const std::vector<int>& off,
-                              const std::vector<int>& col, const std::vector<int>& levptr, bool forward, int band,
-                              size_t lds_cap, size_t vsize) {
+inline TileLayout tile_layout(int n, const std::vector<int>& order, const std::vector<int>& off, const int* col,
+                              const std::vector<int>& levptr, bool forward, int band, size_t lds_cap,
+                              size_t vsize) {
     const int nlev = (int)levptr.size() - 1;
     band = std::max(1, std::min(band, kTileMaxSteps));
     std::vector<int> lev_of_row(n);
