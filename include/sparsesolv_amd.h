@@ -258,8 +258,9 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* op
  * standard direction with mrtr_direction_fix; or SSGPU_MRTR_SGS:
  * MatSolverMRTR_SGSver.cpp:42-282 with S A S from ssgpu_factorize(...,
  * SSGPU_MRTR_SGS, ...)): systems join with ssgpu_batch_add
- * (the matrix prepared by ssgpu_factorize for that solver; no diagonal
- * scaling for the IC solvers, SGS-MRTR always scales;
+ * (the matrix prepared by ssgpu_factorize for that solver, and not
+ * refactorized, updated or destroyed while any of its systems is in the
+ * session; no diagonal scaling for the IC solvers, SGS-MRTR always scales;
  * d_b and d_x device vectors of the matrix's n entries that stay valid until
  * the system's result is polled; d_x is the start vector when
  * opt->is_init == 0 and receives the solution) and enter the loop at the
