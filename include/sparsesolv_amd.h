@@ -230,16 +230,20 @@ int ssgpu_solve_factored_noalloc(ssgpu_context ctx, ssgpu_matrix m, int solver, 
                                  double conv_cri, int max_ite, double normB, const void* d_b, void* d_x,
                                  double* log, int log_cap, ssgpu_result* res);
 
-/* Block ICCG / COCG: nrhs (1..32) right-hand sides against the factor of
- * ssgpu_factorize in one pass of kernels per iteration (the sweeps carry
- * every column through the dependency chain at once).  d_B and d_X hold
- * nrhs vectors of n entries one after the other (entry i of vector j at
- * j * n + i); normB[j] as for
- * ssgpu_solve; res[j] per column (no residual log).  Column j returns the
- * same x, iteration count and residual as ssgpu_solve_factored on that
- * column alone.  No counterpart in the reference, which solves one
- * right-hand side per call (MatSolverBase.hpp:142-150). */
-int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, const ssgpu_options* opt, double conv_cri,
+/* Block solve: nrhs (1..32) right-hand sides of one system against the
+ * preparation of ssgpu_factorize for `solver` (the IC factor for SSGPU_ICCG
+ * and SSGPU_MRTR_IC, S A S and its triangles for SSGPU_MRTR_SGS), in one
+ * pass of kernels per iteration: the sweeps carry every column through the
+ * dependency chain at once and the SpMV reads the matrix once for all
+ * columns.  d_B and d_X hold nrhs vectors of n entries one after the other
+ * (entry i of vector j at j * n + i); normB[j] as for ssgpu_solve; res[j]
+ * per column (no residual log).  Column j returns the same x, iteration
+ * count, residuals and refinement passes as ssgpu_solve_factored with the
+ * same solver on that column alone.  No counterpart in the reference, which
+ * solves one right-hand side per call (MatSolverBase.hpp:142-150); the
+ * iterations are those of MatSolverICCG.cpp:257-318,
+ * MatSolverMRTR_ICver.cpp:270-360 and MatSolverMRTR_SGSver.cpp:182-262. */
+int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix m, int solver, const ssgpu_options* opt, double conv_cri,
                       int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X,
                       ssgpu_result* res);
 

@@ -86,7 +86,7 @@ def lib():
         L.ssgpu_factorize.argtypes = [V, I, D, I, P(D), P(I)]
         L.ssgpu_solve_factored.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
         L.ssgpu_solve_factored_noalloc.argtypes = [V, V, I, P(Options), D, I, D, V, V, V, I, P(Result)]
-        L.ssgpu_solve_block.argtypes = [V, V, P(Options), D, I, I, V, V, V, P(Result)]
+        L.ssgpu_solve_block.argtypes = [V, V, I, P(Options), D, I, I, V, V, V, P(Result)]
         L.ssgpu_batch_create.argtypes = [V, I, I, I, P(V)]
         L.ssgpu_batch_add.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
         L.ssgpu_batch_add_host.argtypes = [V, V, P(Options), D, I, D, V, V, ctypes.c_int64]
@@ -327,15 +327,17 @@ class Matrix:
         check(st, allow=(OK, NOT_CONVERGED))
         return res
 
-    def solve_block(self, ctx: "Context", d_B: int, d_X: int, normB, *, conv_cri=1e-8, max_ite=10000, opts=None):
-        """Block ICCG/COCG over k = len(normB) right-hand sides stored one after the other in d_B (a k x n
-        C-ordered device array); returns one Result per right-hand side."""
+    def solve_block(self, ctx: "Context", d_B: int, d_X: int, normB, *, solver=ICCG, conv_cri=1e-8, max_ite=10000,
+                    opts=None):
+        """Block solve (ICCG/COCG, IC-MRTR or SGS-MRTR, against the preparation of factorize for that
+        solver) over k = len(normB) right-hand sides stored one after the other in d_B (a k x n C-ordered
+        device array); returns one Result per right-hand side, each equal to solve_factored's."""
         o = opts or default_options()
         k = len(normB)
         nb = np.ascontiguousarray(normB, dtype=np.float64)
         res = (Result * k)()
-        st = lib().ssgpu_solve_block(ctx.h, self.h, ctypes.byref(o), conv_cri, max_ite, k, nb.ctypes.data, d_B, d_X,
-                                     res)
+        st = lib().ssgpu_solve_block(ctx.h, self.h, int(solver), ctypes.byref(o), conv_cri, max_ite, k,
+                                     nb.ctypes.data, d_B, d_X, res)
         check(st, allow=(OK, NOT_CONVERGED))
         return list(res)
 

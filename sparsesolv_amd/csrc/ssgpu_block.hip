@@ -178,6 +178,71 @@ __global__ void k_cg_pupdate_cols(int n, int ld, const T* __restrict__ beta, con
         p[o + i] = Num<T>::add(Num<T>::mul(p[o + i], be), z[o + i]);
 }
 
+// one partial per block and column of (a_c . b_c, a_c . c_c, a_c . d_c), as
+// k_dot3 forms them (the third zero without d)
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_dot3_cols(int n, int ld, const T* __restrict__ a, const T* __restrict__ b,
+                                                   const T* __restrict__ cc, const T* __restrict__ d,
+                                                   double* __restrict__ partial) {
+    constexpr int P = BParts<T>::K;
+    const int c = blockIdx.y, kb = gridDim.y;
+    const size_t o = (size_t)c * ld;
+    double acc[3 * P] = {};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T ai = a[o + i];
+        BParts<T>::put(acc, Num<T>::mul(ai, b[o + i]));
+        BParts<T>::put(acc + P, Num<T>::mul(ai, cc[o + i]));
+        if (d) BParts<T>::put(acc + 2 * P, Num<T>::mul(ai, d[o + i]));
+    }
+    block_sum_multi<3 * P>(acc, partial + ((size_t)blockIdx.x * kb + c) * 3 * P);
+}
+
+// MRTR updates of the active columns, column c with the scalars sc[c]
+// (t), sc[kb + c] (zeta), sc[2 kb + c] (eta).  SGS: the Eisenstat form
+// (k_sgs_update); else IC-MRTR's (k_mrtr_update, fix: keep p's recurrence).
+// Partials of |r_c|^2.
+template <typename T, bool SGS>
+__global__ __launch_bounds__(kBS) void k_mrtr_update_cols(int n, int ld, int fix, const T* __restrict__ sc,
+                                                          const int* __restrict__ act, const T* __restrict__ u,
+                                                          const T* __restrict__ v, T* __restrict__ p,
+                                                          T* __restrict__ x, T* __restrict__ y, T* __restrict__ r,
+                                                          double* __restrict__ partial) {
+    const int c = blockIdx.y, kb = gridDim.y;
+    if (!act[c]) return;   // its partial is not read
+    const T t = sc[c], zeta = sc[kb + c], eta = sc[2 * kb + c];
+    const size_t o = (size_t)c * ld;
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T ui = u[o + i];
+        T pi;
+        if (SGS) pi = Num<T>::add(Num<T>::mul(p[o + i], t), ui);
+        else pi = fix ? Num<T>::add(ui, Num<T>::mul(t, p[o + i])) : Num<T>::add(ui, Num<T>::mul(t, ui));
+        p[o + i] = pi;
+        x[o + i] = Num<T>::add(x[o + i], Num<T>::mul(zeta, pi));
+        const T yi = Num<T>::add(Num<T>::mul(y[o + i], eta), Num<T>::mul(zeta, v[o + i]));
+        y[o + i] = yi;
+        const T ri = Num<T>::sub(r[o + i], yi);
+        r[o + i] = ri;
+        acc[0] += Num<T>::abs2(ri);
+    }
+    block_sum_multi<1>(acc, partial + (size_t)blockIdx.x * kb + c);
+}
+
+// z_c = z_c eta_c + zeta_c w_c; u_c -= z_c for the active columns (k_mrtr_update2)
+template <typename T>
+__global__ void k_mrtr_update2_cols(int n, int ld, const T* __restrict__ sc, const int* __restrict__ act,
+                                    const T* __restrict__ w, T* __restrict__ z, T* __restrict__ u) {
+    const int c = blockIdx.y, kb = gridDim.y;
+    if (!act[c]) return;
+    const T zeta = sc[kb + c], eta = sc[2 * kb + c];
+    const size_t o = (size_t)c * ld;
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
+        const T zi = Num<T>::add(Num<T>::mul(z[o + i], eta), Num<T>::mul(zeta, w[o + i]));
+        z[o + i] = zi;
+        u[o + i] = Num<T>::sub(u[o + i], zi);
+    }
+}
+
 inline int grid_of(int64_t n, int cap) {
     int64_t g = (n + kBS - 1) / kBS;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
@@ -209,6 +274,27 @@ void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta,
     hipLaunchKernelGGL(k_cg_pupdate_cols<T>, dim3(grid_of(n, 4096), kb), dim3(kBS), 0, s, n, ld, beta, act, z, p);
 }
 
+template <typename T>
+void launch_dot3_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, const T* c, const T* d,
+                      double* partial) {
+    hipLaunchKernelGGL(k_dot3_cols<T>, dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, a, b, c, d, partial);
+}
+template <typename T>
+void launch_mrtr_update_cols(hipStream_t s, int kb, int n, int ld, bool sgs, int fix, const T* sc, const int* act,
+                             const T* u, const T* v, T* p, T* x, T* y, T* r, double* partial) {
+    if (sgs)
+        hipLaunchKernelGGL((k_mrtr_update_cols<T, true>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, fix, sc,
+                           act, u, v, p, x, y, r, partial);
+    else
+        hipLaunchKernelGGL((k_mrtr_update_cols<T, false>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, fix,
+                           sc, act, u, v, p, x, y, r, partial);
+}
+template <typename T>
+void launch_mrtr_update2_cols(hipStream_t s, int kb, int n, int ld, const T* sc, const int* act, const T* w, T* z,
+                              T* u) {
+    hipLaunchKernelGGL(k_mrtr_update2_cols<T>, dim3(grid_of(n, 4096), kb), dim3(kBS), 0, s, n, ld, sc, act, w, z, u);
+}
+
 #define SSGPU_INST(T)                                                                                              \
     template void launch_spmv_cols<T>(hipStream_t, int, int, int, const int*, const int*, const int*, const T*,     \
                                       const T*, T*, const T*, double*);                                         \
@@ -216,7 +302,11 @@ void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta,
     template void launch_norm2_cols<T>(hipStream_t, int, int, int, const T*, double*);                            \
     template void launch_cg_update_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, const T*, T*,  \
                                            T*, double*);                                                        \
-    template void launch_cg_pupdate_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, T*);
+    template void launch_cg_pupdate_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, T*);      \
+    template void launch_dot3_cols<T>(hipStream_t, int, int, int, const T*, const T*, const T*, const T*, double*); \
+    template void launch_mrtr_update_cols<T>(hipStream_t, int, int, int, bool, int, const T*, const int*, const T*, \
+                                             const T*, T*, T*, T*, T*, double*);                                \
+    template void launch_mrtr_update2_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, T*, T*);
 SSGPU_INST(double)
 SSGPU_INST(zc)
 #undef SSGPU_INST

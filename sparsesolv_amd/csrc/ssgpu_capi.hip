@@ -1363,7 +1363,9 @@ template <typename T> struct BlockIccg {
         return b.as<T>();
     }
     double* bpartial() {
-        const size_t need = sizeof(double) * 64 * (size_t)std::max(m.nblk, kMaxReduceBlocks);
+        // per SpMV block at most kb * P = 64 dot parts; per reduction block
+        // at most 3 kb P = 192 (the MRTR three-dot phase)
+        const size_t need = sizeof(double) * std::max(64 * (size_t)m.nblk, 192 * (size_t)kMaxReduceBlocks);
         if (c->d_bpartial.bytes < need) c->d_bpartial.alloc(c, need);
         return c->d_bpartial.as<double>();
     }
@@ -1376,9 +1378,12 @@ template <typename T> struct BlockIccg {
         if (!c->h_bscal) SSGPU_HIP(hipHostMalloc(&c->h_bscal, 4096, hipHostMallocDefault));
         (void)scal_buf(c);
     }
-    // finalize kb*P partial sums over nb blocks and read them back (synchronizes)
-    std::vector<double> fetch(int nb) {
-        const int K = kb * P;
+    // finalize kb*P*q partial sums over nb blocks (q per-column sums of
+    // P parts each) and read them back (synchronizes)
+    std::vector<double> fetch(int nb, int q = 1) { return fetch_k(nb, kb * P * q); }
+    // the per-column squared norms (one part per column)
+    std::vector<double> fetch_norms(int nb) { return fetch_k(nb, kb); }
+    std::vector<double> fetch_k(int nb, int K) {
         launch_finalize(s, nb, K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
         std::vector<double> out(K);
         const double* h = read_scalars(c);
@@ -1390,15 +1395,17 @@ template <typename T> struct BlockIccg {
         for (int j = 0; j < kb; ++j) out[j] = Host<T>::from(&v[(size_t)j * P]);
         return out;
     }
-    // per-column scalars and active flags to the device (one copy)
+    // per-column scalars (at most kActOff bytes: three per column) and active
+    // flags to the device (one copy)
+    static constexpr size_t kActOff = 3072;
     void put_scalars(const std::vector<T>& a, const std::vector<int>& act) {
         char* h = static_cast<char*>(c->h_bscal);
-        std::memcpy(h, a.data(), sizeof(T) * kb);
-        std::memcpy(h + 1024, act.data(), sizeof(int) * kb);
-        SSGPU_HIP(hipMemcpyAsync(c->d_bscal.p, h, 1024 + sizeof(int) * kb, hipMemcpyHostToDevice, s));
+        std::memcpy(h, a.data(), sizeof(T) * a.size());
+        std::memcpy(h + kActOff, act.data(), sizeof(int) * kb);
+        SSGPU_HIP(hipMemcpyAsync(c->d_bscal.p, h, kActOff + sizeof(int) * kb, hipMemcpyHostToDevice, s));
     }
     const T* d_scalars() { return c->d_bscal.as<T>(); }
-    const int* d_act() { return reinterpret_cast<const int*>(static_cast<char*>(c->d_bscal.p) + 1024); }
+    const int* d_act() { return reinterpret_cast<const int*>(static_cast<char*>(c->d_bscal.p) + kActOff); }
 
     void ic_apply(const T* r, T* z, T* tmp) {
         launch_sf_prepare<T>(s, n * kb, tmp, z, S.tickets());
@@ -1425,6 +1432,26 @@ template <typename T> struct BlockIccg {
                                      m.sfUic.ecol.as<int>(), m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
                                      m.diagD.as<T>(), z);
         }
+    }
+
+    // the plain sweeps of SGS-MRTR on S A S's triangles: z = fr(r) (forward,
+    // divide by the row's last entry) or bc(r) (backward, divide by the first)
+    void sgs_sweep(bool forward, const T* r, T* z) {
+        launch_sf_prepare<T>(s, n * kb, z, nullptr, S.tickets());
+        int* tk = btickets();
+        launch_fill<int>(s, 64, tk, 0);
+        const int ng = kb / kc;
+        const int grid = m.block_grid > 0 ? m.block_grid : std::min(m.sweep_grid, std::max(32, 512 / ng));
+        const SfCsr& sf = forward ? m.sfL : m.sfUs;
+        const int mode = forward ? FWD_DIV_LAST : BWD_DIV_FIRST;
+        if (S.o.factor_precision == SSGPU_FACTOR_F32)
+            launch_sweep_block<T, F>(s, mode, sf.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit, sf.srow,
+                                     sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval32.as<F>(),
+                                     sf.dval32.as<F>(), r, nullptr, nullptr, z);
+        else
+            launch_sweep_block<T, T>(s, mode, sf.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit, sf.srow,
+                                     sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval.as<T>(),
+                                     sf.dval.as<T>(), r, nullptr, nullptr, z);
     }
 
     struct Col {
@@ -1533,21 +1560,186 @@ template <typename T> struct BlockIccg {
                 if (!col[j].conv) launch_copy<T>(s, n, BEST + (size_t)j * n, X + (size_t)j * n);
         return col;
     }
+
+    // the prologue of Solver::prologue per column, from the norms of the
+    // first residuals; false when no column goes on
+    bool prologue(std::vector<Col>& col, double conv_cri, const std::vector<double>& normB,
+                  const std::vector<double>& pure2) {
+        constexpr double kSmallAbs = 1.0e-20;
+        bool any = false;
+        for (int j = 0; j < kb; ++j) {
+            Col& q = col[j];
+            const double pure = std::sqrt(pure2[j]);
+            const double nB = normB[j];
+            if (nB <= 1.0e-30 && pure <= kSmallAbs) { q.active = false; q.conv = true; continue; }
+            q.abs_conv = (nB * conv_cri * 0.9 < kSmallAbs ? kSmallAbs : nB * conv_cri * 0.9);
+            if (pure / nB < conv_cri * 0.5 || pure < q.abs_conv * 0.5) { q.active = false; q.conv = true; continue; }
+            if (S.o.conv_normalize_type == 1) q.normalizer = pure;
+            else if (S.o.conv_normalize_type == 2) q.normalizer = S.o.conv_normalize_const;
+            else q.normalizer = nB;
+            any = true;
+        }
+        return any;
+    }
+
+    // IC-MRTR (Solver::mrtr_main) or, with sgs, the SGS-MRTR iteration
+    // (Solver::sgs_main) per column.  B, X: kb columns; for SGS they are the
+    // scaled right-hand sides and iterates and normB the norms of the scaled
+    // right-hand sides, as solve_factored hands them to sgs_main.
+    std::vector<Col> run_mrtr(bool sgs, double conv_cri, int max_ite, const std::vector<double>& normB, const T* B,
+                              T* X) {
+        const ssgpu_options& o = S.o;
+        ensure_scal();
+        bpartial();
+        // P, R (Rd), U, X (the iterate), Y, Zv (IC) and V, W (IC; ARd, tmp for SGS), tmp
+        T *P_ = bvec(20), *R = bvec(21), *U = bvec(22), *Xi = bvec(23), *Y = bvec(24), *Zv = bvec(25);
+        T *V = bvec(28), *W = bvec(29), *TMP = bvec(30);
+        T* BEST = o.is_save_best ? bvec(31) : nullptr;
+        const int64_t nk = (int64_t)n * kb;
+        const T* av = sgs ? m.scaledA.as<T>() : m.val.as<T>();
+        const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
+        std::vector<Col> col(kb);
+        launch_fill<T>(s, nk, P_, Host<T>::zero());
+        if (o.is_init) {
+            launch_fill<T>(s, nk, X, Host<T>::zero());
+            launch_fill<T>(s, nk, Xi, Host<T>::zero());
+        } else {
+            launch_copy<T>(s, nk, X, Xi);
+        }
+        if (BEST) launch_fill<T>(s, nk, BEST, Host<T>::zero());
+        // the first residual: MatSolverMRTR_ICver.cpp:213-216 and
+        // MatSolverMRTR_SGSver.cpp:112-114 subtract A x only when is_init
+        T* r0 = sgs ? TMP : R;
+        if (o.is_init) {
+            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, X, V, nullptr,
+                                nullptr);
+            launch_residual<T>(s, (int)nk, B, V, r0);
+        } else {
+            launch_residual<T>(s, (int)nk, B, nullptr, r0);
+        }
+        if (sgs) sgs_sweep(true, TMP, R);   // Rd = fr(b - A x)
+        launch_norm2_cols<T>(s, kb, n, n, R, c->d_bpartial.as<double>());
+        if (!prologue(col, conv_cri, normB, fetch_norms(reduce_blocks(n)))) return col;
+        if (!sgs) ic_apply(R, U, TMP);
+        launch_negate<T>(s, (int)nk, R, Y);
+        if (!sgs) ic_apply(Y, Zv, TMP);
+        std::vector<T> zeta(kb, Solver::one<T>()), zeta_old(kb, Solver::one<T>()), eta(kb, Host<T>::zero()),
+            nu(kb, Solver::one<T>());
+        std::vector<T> sc(3 * (size_t)kb, Host<T>::zero());
+        std::vector<int> act(kb, 0);
+        auto any_active = [&]() {
+            for (auto& q : col) if (q.active) return true;
+            return false;
+        };
+        // v = the preconditioned operator on u's direction, w its product
+        // with the preconditioner (IC); for SGS both are ARd
+        T *dv = V, *dw = W;
+        for (int it = 0; it < max_ite && any_active(); ++it) {
+            if (sgs) {
+                sgs_sweep(false, R, U);   // U = bc(Rd)
+                if (f32) {
+                    launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U,
+                                        TMP, nullptr, nullptr);
+                    sgs_sweep(true, TMP, V);
+                } else {
+                    launch_sub<T>(s, (int)nk, R, U, TMP);
+                    sgs_sweep(true, TMP, V);
+                    launch_add_inplace<T>(s, (int)nk, U, V);
+                }
+                dv = V;
+                dw = V;
+            } else {
+                launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U, V,
+                                    nullptr, nullptr);
+                ic_apply(V, W, TMP);
+            }
+            launch_dot3_cols<T>(s, kb, n, n, dw, R, dv, it == 0 ? nullptr : Y, c->d_bpartial.as<double>());
+            const std::vector<double> d3 = fetch(reduce_blocks(n), 3);
+            for (int j = 0; j < kb; ++j) {
+                act[j] = col[j].active ? 1 : 0;
+                if (!col[j].active) continue;
+                const double* d = &d3[(size_t)j * 3 * P];
+                const T w_r = Host<T>::from(d), v_w = Host<T>::from(d + P);
+                if (it == 0) {
+                    zeta[j] = Host<T>::div(w_r, v_w);
+                    zeta_old[j] = zeta[j];
+                    eta[j] = Host<T>::zero();
+                } else {
+                    const T w_y = Host<T>::from(d + 2 * P);
+                    const T t0 = hdiv(1.0, nu[j] * v_w - w_y * w_y);
+                    zeta[j] = nu[j] * w_r * t0;
+                    eta[j] = -1.0 * w_y * w_r * t0;
+                }
+                nu[j] = zeta[j] * w_r;
+                sc[j] = Host<T>::div(eta[j] * zeta_old[j], zeta[j]);
+                sc[kb + j] = zeta[j];
+                sc[2 * kb + j] = eta[j];
+                zeta_old[j] = zeta[j];
+            }
+            put_scalars(sc, act);
+            launch_mrtr_update_cols<T>(s, kb, n, n, sgs, o.mrtr_direction_fix, d_scalars(), d_act(), U, dv, P_, Xi, Y,
+                                       R, c->d_bpartial.as<double>());
+            const std::vector<double> pn = fetch_norms(reduce_blocks(n));
+            for (int j = 0; j < kb; ++j) {   // Solver::monitor, per column
+                Col& q = col[j];
+                if (!q.active) continue;
+                q.iterations = it + 1;
+                const double p = std::sqrt(pn[j]);
+                const double normR = p / q.normalizer;
+                q.final_res = normR;
+                if (normR < conv_cri || p < q.abs_conv) { q.active = false; q.conv = true; continue; }
+                if (normR < q.best) {
+                    q.best = normR;
+                    if (BEST) launch_copy<T>(s, n, Xi + (size_t)j * n, BEST + (size_t)j * n);
+                }
+                if (o.diverge_judge_type == 1) {
+                    if (normR < q.best * o.bad_div_val) q.bad = 0;
+                    if (normR >= q.best * o.bad_div_val) q.bad++;
+                    if (q.bad >= o.bad_div_count_thres) { q.active = false; continue; }
+                }
+            }
+            if (!any_active()) break;
+            if (!sgs) {
+                for (int j = 0; j < kb; ++j) act[j] = col[j].active ? 1 : 0;
+                put_scalars(sc, act);
+                launch_mrtr_update2_cols<T>(s, kb, n, n, d_scalars(), d_act(), W, Zv, U);
+            }
+        }
+        launch_copy<T>(s, nk, Xi, X);
+        if (BEST)
+            for (int j = 0; j < kb; ++j)
+                if (!col[j].conv) launch_copy<T>(s, n, BEST + (size_t)j * n, X + (size_t)j * n);
+        return col;
+    }
 };
 
-// Block solve entry: copy to kb padded columns, run, copy back, true residuals.
+// Block solve entry: copy to kb padded columns, run, copy back, true
+// residuals (and, for an FP32 factor, the refinement passes of
+// solve_factored on each column that needs them).  Column j returns the bits
+// of solve_factored on right-hand side j.
 template <typename T>
-int solve_block(Context* c, Matrix& m, const ssgpu_options& o, double conv_cri, int max_ite, int nrhs,
+int solve_block(Context* c, Matrix& m, int solver, const ssgpu_options& o, double conv_cri, int max_ite, int nrhs,
                 const double* normB, const T* b, T* x, ssgpu_result* res) {
-    if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
-    if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
-        throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
-    if (o.is_diag_scale) throw Error{UNSUPPORTED, "diagonal scaling is not available on a cached factor"};
-    if (!m.sfL.kmax || !m.sfUic.kmax || m.sfL.kmax > 32 || m.sfUic.kmax > 32)
-        throw Error{UNSUPPORTED, "block solves need rows of at most 32 off-diagonal entries"};
+    const bool sgs = solver == SSGPU_MRTR_SGS;
+    if (solver != SSGPU_ICCG && solver != SSGPU_MRTR_IC && !sgs) throw Error{BAD_ARG, "unknown solver"};
+    if (sgs) {
+        if (m.prep_kind != 2) throw Error{BAD_ARG, "no SGS preparation: call ssgpu_factorize with SSGPU_MRTR_SGS"};
+        if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the SGS preparation was built without its FP32 copy"};
+        if (!m.sfL.kmax || !m.sfUs.kmax || m.sfL.kmax > 32 || m.sfUs.kmax > 32)
+            throw Error{UNSUPPORTED, "block solves need rows of at most 32 off-diagonal entries"};
+    } else {
+        if (m.prep_kind != 1) throw Error{BAD_ARG, "no IC factor: call ssgpu_factorize first"};
+        if (o.factor_precision == SSGPU_FACTOR_F32 && m.prep_precision != SSGPU_FACTOR_F32)
+            throw Error{BAD_ARG, "the factor was built without its FP32 copy"};
+        if (o.is_diag_scale) throw Error{UNSUPPORTED, "diagonal scaling is not available on a cached factor"};
+        if (!m.sfL.kmax || !m.sfUic.kmax || m.sfL.kmax > 32 || m.sfUic.kmax > 32)
+            throw Error{UNSUPPORTED, "block solves need rows of at most 32 off-diagonal entries"};
+    }
     int kb = 1;
     while (kb < nrhs) kb *= 2;
     if (kb > 32) throw Error{BAD_ARG, "at most 32 right-hand sides per block solve"};
+    if ((int64_t)m.n * kb >= INT_MAX) throw Error{BAD_ARG, "n x the padded column count must stay below 2^31"};
     Solver S(m, c, o);
     BlockIccg<T> B(S, kb);
     hipStream_t s = c->stream;
@@ -1566,8 +1758,31 @@ int solve_block(Context* c, Matrix& m, const ssgpu_options& o, double conv_cri, 
         launch_fill<T>(s, npad, Xp + nr, Host<T>::zero());
     }
     std::vector<double> nB(kb, 0.0);
-    for (int j = 0; j < nrhs; ++j) nB[j] = normB[j];
-    auto col = B.run(conv_cri, max_ite, nB, Bp, Xp);
+    std::vector<typename BlockIccg<T>::Col> col;
+    if (sgs) {
+        // as solve_factored around sgs_main: b~ = s o b, x~ = x / s for a
+        // warm start, the norms of b~ for the checks, x = s o x~ at the end
+        T* Bs = B.bvec(32);
+        T* Xs = B.bvec(33);
+        launch_copy<T>(s, (int64_t)n * kb, Bp, Bs);
+        launch_copy<T>(s, (int64_t)n * kb, Xp, Xs);
+        for (int j = 0; j < kb; ++j) {
+            launch_mul_scale<T>(s, n, m.sc.as<T>(), Bs + (size_t)j * n);
+            if (!o.is_init) launch_div_scale<T>(s, n, m.sc.as<T>(), Xs + (size_t)j * n);
+        }
+        B.ensure_scal();
+        B.bpartial();
+        launch_norm2_cols<T>(s, kb, n, n, Bs, c->d_bpartial.as<double>());
+        const std::vector<double> n2 = B.fetch_norms(reduce_blocks(n));
+        for (int j = 0; j < kb; ++j) nB[j] = std::sqrt(n2[j]);
+        col = B.run_mrtr(true, conv_cri, max_ite, nB, Bs, Xs);
+        for (int j = 0; j < kb; ++j) launch_mul_scale<T>(s, n, m.sc.as<T>(), Xs + (size_t)j * n);
+        launch_copy<T>(s, (int64_t)n * kb, Xs, Xp);
+    } else {
+        for (int j = 0; j < nrhs; ++j) nB[j] = normB[j];
+        col = solver == SSGPU_ICCG ? B.run(conv_cri, max_ite, nB, Bp, Xp)
+                                   : B.run_mrtr(false, conv_cri, max_ite, nB, Bp, Xp);
+    }
     launch_copy<T>(s, nr, Xp, x);
     // true residuals ||b_j - A x_j|| / normB_j
     T* Rt = B.bvec(21);
@@ -1578,6 +1793,22 @@ int solve_block(Context* c, Matrix& m, const ssgpu_options& o, double conv_cri, 
     launch_norm2_cols<T>(s, kb, n, n, Rt, c->d_bpartial.as<double>());
     launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
     SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * kb, hipMemcpyDeviceToHost, s));
+    SSGPU_HIP(hipEventRecord(e2, s));
+    SSGPU_HIP(hipEventSynchronize(e2));
+    S.check_sweep_error();
+    std::vector<double> true_res(nrhs);
+    for (int j = 0; j < nrhs; ++j) true_res[j] = std::sqrt(c->h_scal[j]) / (normB[j] > 0.0 ? normB[j] : 1.0);
+    // an FP32 factor's refinement passes, column by column, exactly as
+    // solve_factored runs them (finish_residual recomputes the same true
+    // residual first)
+    const bool refinable = o.factor_precision == SSGPU_FACTOR_F32 &&
+                           (solver == SSGPU_ICCG || (solver == SSGPU_MRTR_IC && o.mrtr_direction_fix));
+    std::vector<int> refinements(nrhs, 0), refine_its(nrhs, 0);
+    if (refinable)
+        for (int j = 0; j < nrhs; ++j)
+            if (col[j].conv && true_res[j] > conv_cri)
+                true_res[j] = finish_residual<T>(S, solver, o, true, true, conv_cri, max_ite, normB[j],
+                                                 b + (size_t)j * n, x + (size_t)j * n, refinements[j], refine_its[j]);
     SSGPU_HIP(hipEventRecord(e2, s));
     SSGPU_HIP(hipEventSynchronize(e2));
     S.check_sweep_error();
@@ -1593,10 +1824,12 @@ int solve_block(Context* c, Matrix& m, const ssgpu_options& o, double conv_cri, 
         r.status = col[j].conv ? OK : NOT_CONVERGED;
         r.iterations = col[j].iterations;
         r.final_residual = col[j].final_res;
-        r.accel_used = m.factor_accel;
-        r.pivot_fixes = m.factor_pivot_fixes;
+        r.accel_used = sgs ? 0.0 : m.factor_accel;
+        r.pivot_fixes = sgs ? 0 : m.factor_pivot_fixes;
         r.iterate_ms = t_it;
-        r.true_residual = std::sqrt(c->h_scal[j]) / (normB[j] > 0.0 ? normB[j] : 1.0);
+        r.true_residual = true_res[j];
+        r.refinements = refinements[j];
+        r.refine_iterations = refine_its[j];
         all_conv &= r.converged;
     }
     return all_conv ? OK : NOT_CONVERGED;
@@ -2215,10 +2448,10 @@ int ssgpu_solve_factored_noalloc(ssgpu_context ctx, ssgpu_matrix mh, int solver,
     SSGPU_GUARD_END
 }
 
-int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix mh, const ssgpu_options* opt, double conv_cri, int max_ite,
-                      int nrhs, const double* normB, const void* d_B, void* d_X, ssgpu_result* res) {
+int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix mh, int solver, const ssgpu_options* opt, double conv_cri,
+                      int max_ite, int nrhs, const double* normB, const void* d_B, void* d_X, ssgpu_result* res) {
     if (!ctx || !mh || !opt || !normB || !d_B || !d_X || !res || nrhs < 1 || nrhs > 32) return SSGPU_BAD_ARG;
-    const int ck = check_opts(opt, SSGPU_ICCG);
+    const int ck = check_opts(opt, solver);
     if (ck != SSGPU_OK) return ck;
     SSGPU_GUARD_BEGIN
     Matrix& m = mh->m;
@@ -2233,9 +2466,10 @@ int ssgpu_solve_block(ssgpu_context ctx, ssgpu_matrix mh, const ssgpu_options* o
         d_X = pio->xp.p;
     }
     const int st = m.dtype == SSGPU_R64
-                       ? solve_block<double>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const double*)d_B,
-                                             (double*)d_X, res)
-                       : solve_block<zc>(c, m, *opt, conv_cri, max_ite, nrhs, normB, (const zc*)d_B, (zc*)d_X, res);
+                       ? solve_block<double>(c, m, solver, *opt, conv_cri, max_ite, nrhs, normB,
+                                             (const double*)d_B, (double*)d_X, res)
+                       : solve_block<zc>(c, m, solver, *opt, conv_cri, max_ite, nrhs, normB, (const zc*)d_B,
+                                         (zc*)d_X, res);
     if (pio) pio->finish();
     return st;
     SSGPU_GUARD_END

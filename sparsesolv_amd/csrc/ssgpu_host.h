@@ -59,8 +59,9 @@ struct Context {
 // Device scalars of a context: reduction results in the first slots, and in
 // the last slot the flag a sync-free sweep sets when it exceeds its spin
 // limit.  Every scalar read-back copies the whole block, so a solve sees the
-// flag at its next convergence check and stops there.
-constexpr int kScalSlots = 128;
+// flag at its next convergence check and stops there.  The largest read-back
+// is a block MRTR's three complex dots for 32 columns (192 values).
+constexpr int kScalSlots = 256;
 constexpr int kErrInt = 2 * (kScalSlots - 1);   // the flag, as an int index into the block
 DevBuf& scal_buf(Context* c);
 // copy the scalar block to the pinned host copy and synchronize; a set sweep

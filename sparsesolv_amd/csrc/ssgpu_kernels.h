@@ -195,6 +195,17 @@ void launch_cg_update_cols(hipStream_t s, int kb, int n, int ld, const T* alpha,
                            const T* q, T* x, T* r, double* partial);
 template <typename T>
 void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta, const int* act, const T* z, T* p);
+// partials: kb * 3 * parts per block, as launch_dot3 per column (d may be null)
+template <typename T>
+void launch_dot3_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, const T* c, const T* d,
+                      double* partial);
+// sc: t[kb], zeta[kb], eta[kb]; sgs: the SGS-MRTR update, else IC-MRTR's
+template <typename T>
+void launch_mrtr_update_cols(hipStream_t s, int kb, int n, int ld, bool sgs, int fix, const T* sc, const int* act,
+                             const T* u, const T* v, T* p, T* x, T* y, T* r, double* partial);
+template <typename T>
+void launch_mrtr_update2_cols(hipStream_t s, int kb, int n, int ld, const T* sc, const int* act, const T* w, T* z,
+                              T* u);
 // --- batched solves (many independent systems, one launch per phase) ----------------
 // Each system keeps its own vectors and scalar state; a launch covers every
 // system of the batch whose status equals the launch's `want` (the others are

@@ -25,11 +25,11 @@ context):
   Requests with diagonal scaling on a cached IC factor (which the library
   refuses there) and every request with batching off are solved one at a
   time on the builder that admitted them.
-* Several ICCG requests of one system with the same settings waiting in the
-  queue together are taken by one builder as one block solve
-  (ssgpu_solve_block, up to 32 right-hand sides): one dependency chain per
-  sweep and one pass over the matrix per SpMV for all of them, where the
-  batch would walk a chain per request.
+* Several requests of one system with the same solver and settings waiting
+  in the queue together are taken by one builder as one block solve
+  (ssgpu_solve_block, up to 32 right-hand sides, for ICCG, IC-MRTR and
+  SGS-MRTR): one dependency chain per sweep and one pass over the matrix per
+  SpMV for all of them, where the batch would walk a chain per request.
 
 Each result is exactly what a solo solve against the same factor returns
 (the batched kernels use the one-system kernels' blocks and reduction trees;
@@ -133,12 +133,17 @@ def _batch_vector_bytes(n: int, cplx: bool, solver: int = capi.ICCG) -> int:
     return (16 if cplx else 8) * n * nvec + 64 * (n // 128 + 1024) + 4096
 
 
-def _block_bytes(n: int, cplx: bool, k: int) -> int:
+def _block_bytes(n: int, cplx: bool, k: int, solver: int = capi.ICCG) -> int:
     """Upper bound of the work arrays of a block solve of k right-hand sides
-    (ssgpu_solve_block): at most 10 arrays of n x k values (eight of the block
-    iteration, two permuted copies for an ABMC-ordered matrix) and the
-    per-column reduction partials."""
-    return 10 * (16 if cplx else 8) * n * k + 512 * (n // 64 + 1024) + (1 << 20)
+    (ssgpu_solve_block), which pads k to a power of two kb: arrays of n x kb
+    values (ICCG 8, IC-MRTR 12, SGS-MRTR 14 with its scaled copies, plus two
+    permuted copies for an ABMC-ordered matrix), the per-column reduction
+    partials, and the one-column pool of an FP32 factor's refinement passes."""
+    kb = 1
+    while kb < k:
+        kb *= 2
+    arrays = {capi.ICCG: 10, capi.MRTR_IC: 14}.get(solver, 16)
+    return arrays * (16 if cplx else 8) * n * kb + 512 * (n // 64 + 1024) + _pool_bytes(n, cplx) + (1 << 20)
 
 
 MAX_BLOCK = 32   # right-hand sides of one block solve (ssgpu_solve_block's limit)
@@ -659,7 +664,7 @@ class Engine:
             return
         r0 = good[0][0]
         n, cplx = e.matrix.n, e.matrix.dtype == np.complex128
-        need = _block_bytes(n, cplx, len(good))
+        need = _block_bytes(n, cplx, len(good), r0.solver)
         self._reserve_wait(need, holding=True, own_ctx=ctx)
         try:
             B = np.stack([np.ascontiguousarray(r.rhs, dtype=e.matrix.dtype) for r, _, _ in good])
@@ -669,8 +674,8 @@ class Engine:
                 d_B = torch.from_numpy(B).to("cuda")
                 d_X = torch.zeros_like(d_B)
             stream.synchronize()
-            res = e.matrix.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=r0.conv_cri,
-                                       max_ite=r0.max_ite, opts=opts)
+            res = e.matrix.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, solver=r0.solver,
+                                       conv_cri=r0.conv_cri, max_ite=r0.max_ite, opts=opts)
             with torch.cuda.stream(stream):
                 X = d_X.cpu().numpy()
             with self.lock:
@@ -713,10 +718,10 @@ class Engine:
                 continue
             # (SGS-MRTR always scales, whatever is_diag_scale says)
             if self.batching and (r.solver == capi.MRTR_SGS or not r.opts.get("is_diag_scale", 0)):
-                # several ICCG requests of one system waiting together share one
-                # block solve (one dependency chain and one matrix pass for all
+                # several requests of one system and solver waiting together share
+                # one block solve (one dependency chain and one matrix pass for all
                 # of them); other requests join the continuous batch of their solver
-                more = self._coalesce(r, key) if r.solver == capi.ICCG else []
+                more = self._coalesce(r, key)
                 if more:
                     with self.lock:
                         e.users += len(more)

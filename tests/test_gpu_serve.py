@@ -225,18 +225,23 @@ def test_serving_batches_requests_together(torch_cuda, ctx, solver):
         assert np.array_equal(r.x, x_solo)
 
 
-@pytest.mark.parametrize("cplx,prec", [(False, capi.FACTOR_F64), (False, capi.FACTOR_F32), (True, capi.FACTOR_F64)],
-                         ids=["real", "real-f32", "complex"])
-def test_serving_block_solves_same_system_requests(torch_cuda, ctx, cplx, prec):
-    """Requests of one system with the same settings queued together share a
-    block solve (one chain, one matrix pass for all); a request with other
-    settings does not join it; each response equals its solo solve."""
+@pytest.mark.parametrize("cplx,prec,solver", [(False, capi.FACTOR_F64, capi.ICCG), (False, capi.FACTOR_F32, capi.ICCG),
+                                              (True, capi.FACTOR_F64, capi.ICCG),
+                                              (False, capi.FACTOR_F64, capi.MRTR_IC),
+                                              (False, capi.FACTOR_F64, capi.MRTR_SGS),
+                                              (False, capi.FACTOR_F32, capi.MRTR_SGS)],
+                         ids=["real", "real-f32", "complex", "mrtr_ic", "mrtr_sgs", "mrtr_sgs-f32"])
+def test_serving_block_solves_same_system_requests(torch_cuda, ctx, cplx, prec, solver):
+    """Requests of one system with the same solver and settings queued
+    together share a block solve (one chain, one matrix pass for all); a
+    request with other settings does not join it; each response equals its
+    solo solve."""
     s = voxel.eddy_current(6, seed=4) if cplx else voxel.curlcurl(9, seed=4)
     bs = [_rhs(s, 300 + k) for k in range(7)]
     reqs = [serve.Request(key="one", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=b, accel=1.1,
-                          precision=prec) for b in bs]
+                          precision=prec, solver=solver) for b in bs]
     reqs.append(serve.Request(key="one", row_ptr=s.row_ptr, col_idx=s.col_idx, values=s.values, rhs=bs[0],
-                              accel=1.1, precision=prec, conv_cri=1e-6))
+                              accel=1.1, precision=prec, solver=solver, conv_cri=1e-6))
     eng = serve.Engine(slots=1)
     first = eng.submit(reqs[0]).result(timeout=120)   # builds the system
     futs = eng.submit_many(reqs[1:])
@@ -246,7 +251,7 @@ def test_serving_block_solves_same_system_requests(torch_cuda, ctx, cplx, prec):
     assert eng.block_solves >= 1
     for r, q in zip(resp, reqs):
         assert r.error is None, r.error
-        x_solo, r_solo = _solo(torch_cuda, ctx, s, q.rhs, capi.ICCG, 1.1, prec, conv_cri=q.conv_cri)
+        x_solo, r_solo = _solo(torch_cuda, ctx, s, q.rhs, solver, 1.1, prec, conv_cri=q.conv_cri)
         assert r.result.iterations == r_solo.iterations
         assert np.array_equal(r.x, x_solo)
 
