@@ -1333,12 +1333,13 @@ int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, do
     return conv ? OK : NOT_CONVERGED;
 }
 
-// Block ICCG / COCG: nrhs right-hand sides against the cached IC factor, in
-// kb >= nrhs padded columns (each stored whole: column c at offset c * n).  Column j runs exactly the solve
-// iccg_main runs on it alone (same kernels per column, same reduction trees,
-// same host recurrences and bookkeeping); a column that converges or
+// Block solves: nrhs right-hand sides against the cached preparation, in
+// kb >= nrhs padded columns (each stored whole: column c at offset c * n).
+// Column j runs exactly the solve iccg_main (run), mrtr_main or sgs_main
+// (run_mrtr) runs on it alone (same kernels per column, same reduction
+// trees, same host recurrences and bookkeeping); a column that converges or
 // diverges stops updating its x while the others go on.
-template <typename T> struct BlockIccg {
+template <typename T> struct BlockSolve {
     Solver& S;
     Context* c;
     Matrix& m;
@@ -1346,7 +1347,7 @@ template <typename T> struct BlockIccg {
     int n, kb, P;
     int kc;   // columns per lane of the block sweeps
     using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
-    explicit BlockIccg(Solver& SS, int kbb) : S(SS), c(SS.c), m(SS.m), s(SS.s), n(SS.m.n), kb(kbb), P(kparts<T>()) {
+    explicit BlockSolve(Solver& SS, int kbb) : S(SS), c(SS.c), m(SS.m), s(SS.s), n(SS.m.n), kb(kbb), P(kparts<T>()) {
         // one column per lane measured best (shortest polling pass; the
         // columns' dependency chains run side by side in separate groups)
         const int lim = sizeof(T) == 8 ? 4 : 2;
@@ -1741,7 +1742,7 @@ int solve_block(Context* c, Matrix& m, int solver, const ssgpu_options& o, doubl
     if (kb > 32) throw Error{BAD_ARG, "at most 32 right-hand sides per block solve"};
     if ((int64_t)m.n * kb >= INT_MAX) throw Error{BAD_ARG, "n x the padded column count must stay below 2^31"};
     Solver S(m, c, o);
-    BlockIccg<T> B(S, kb);
+    BlockSolve<T> B(S, kb);
     hipStream_t s = c->stream;
     const int n = m.n;
     hipEvent_t e1, e2;
@@ -1758,7 +1759,7 @@ int solve_block(Context* c, Matrix& m, int solver, const ssgpu_options& o, doubl
         launch_fill<T>(s, npad, Xp + nr, Host<T>::zero());
     }
     std::vector<double> nB(kb, 0.0);
-    std::vector<typename BlockIccg<T>::Col> col;
+    std::vector<typename BlockSolve<T>::Col> col;
     if (sgs) {
         // as solve_factored around sgs_main: b~ = s o b, x~ = x / s for a
         // warm start, the norms of b~ for the checks, x = s o x~ at the end
