@@ -208,9 +208,17 @@ def test_serving_batches_requests_together(torch_cuda, ctx, solver):
     systems = [voxel.curlcurl(n, seed=8 + n) for n in (10, 7, 12, 9, 6, 11)]
     plan = [(i, _rhs(systems[i], 200 + i)) for i in range(len(systems))]
     eng = serve.Engine(slots=2)
-    futs = eng.submit_many([serve.Request(key=i, row_ptr=systems[i].row_ptr, col_idx=systems[i].col_idx,
-                                          values=systems[i].values, rhs=b, accel=1.1, solver=solver)
-                            for i, b in plan])
+
+    def req(i, b):
+        return serve.Request(key=i, row_ptr=systems[i].row_ptr, col_idx=systems[i].col_idx,
+                             values=systems[i].values, rhs=b, accel=1.1, solver=solver)
+
+    # build every system first (a build takes longer than these small solves,
+    # so requests that wait for their builds would reach the batch one by
+    # one); the measured requests then hit the cache and arrive together
+    for f in eng.submit_many([req(i, systems[i].rhs) for i in range(len(systems))]):
+        assert f.result(timeout=120).error is None
+    futs = eng.submit_many([req(i, b) for i, b in plan])
     resp = [f.result(timeout=120) for f in futs]
     occupancy = list(eng.batches)
     eng.close()
