@@ -452,10 +452,10 @@ def workload_serving(a):
 
 def workload_block(a):
     """32 right-hand sides of the n=95 curl-curl system (the coil moved per
-    column, seeded) as one block solve (ssgpu_solve_block), with the FP64
-    factor and with the FP32 factor: one dependency chain per sweep and one
-    matrix pass per SpMV for all 32 columns.  Reports solves/s and the time
-    per iteration for both factor precisions."""
+    column, seeded) as one block solve (ssgpu_solve_block): shifted ICCG with
+    the FP64 and with the FP32 factor, then IC-MRTR and SGS-MRTR (FP64): one
+    dependency chain per sweep and one matrix pass per SpMV for all 32
+    columns.  Reports solves/s and the time per iteration of each."""
     import torch
     from sparsesolv_amd import capi, voxel
     s = voxel.curlcurl(a.n, seed=0, iron_fraction=0.2, ordering=a.ordering)
@@ -463,19 +463,22 @@ def workload_block(a):
                     for k in range(32)])
     normB = [float(np.linalg.norm(r)) for r in rhs]
     ctx = capi.Context(0)
-    out = {"metric": "block solves/s (32 right-hand sides, one factor), FP64 vs FP32 factor", "unit": "solves/s",
+    out = {"metric": "block solves/s (32 right-hand sides, one system): ICCG FP64 vs FP32 factor, IC-MRTR, "
+                     "SGS-MRTR", "unit": "solves/s",
            "n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded voxel curl-curl system)",
-           "config": {"workload": f"shifted ICCG block of 32, curl-curl n={a.n}, N={s.n}, shift {a.accel}, "
-                                  f"tol {a.tol}"}}
-    for name, prec in (("f64", capi.FACTOR_F64), ("f32_factor", capi.FACTOR_F32)):
+           "config": {"workload": f"block of 32, curl-curl n={a.n}, N={s.n}, shift {a.accel}, tol {a.tol}"}}
+    cases = (("f64", capi.ICCG, capi.FACTOR_F64), ("f32_factor", capi.ICCG, capi.FACTOR_F32),
+             ("mrtr_ic_f64", capi.MRTR_IC, capi.FACTOR_F64), ("sgs_mrtr_f64", capi.MRTR_SGS, capi.FACTOR_F64))
+    for name, solver, prec in cases:
         m = capi.Matrix(ctx, s.row_ptr, s.col_idx, s.values)
-        m.factorize(capi.ICCG, a.accel, prec)
+        m.factorize(capi.ICCG if solver == capi.MRTR_IC else solver, a.accel, prec)
         opts = capi.default_options(accel_factor=a.accel, factor_precision=prec)
         d_B = torch.from_numpy(np.ascontiguousarray(rhs)).cuda()
         d_X = torch.zeros_like(d_B)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = m.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, conv_cri=a.tol, max_ite=20000, opts=opts)
+        res = m.solve_block(ctx, d_B.data_ptr(), d_X.data_ptr(), normB, solver=solver, conv_cri=a.tol,
+                            max_ite=20000, opts=opts)
         el = time.perf_counter() - t0
         its = max(r.iterations for r in res)
         out[name] = {"solves_per_s": 32 / el, "seconds": el, "iterations": its, "ms_per_iteration": 1e3 * el / its,
