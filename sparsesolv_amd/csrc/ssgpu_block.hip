@@ -56,9 +56,12 @@ __device__ inline zc ntl(const zc* p) {
 // y_c = A x_c for the kb columns (column c at offset c * ld); the block's
 // entries are staged in LDS once and every column is summed over them as the
 // one-column kernel sums it (two threads per row: the even- and odd-position
-// chains, y = ta + tb).  With dotv, one partial per SpMV block and column of
-// dotv_c . y_c, row j's term in thread slot j as in the one-column kernel.
-template <typename T>
+// chains, y = ta + tb).  Q columns at a time: their gathers are issued
+// together, each column's sum still in its own order.  With dotv, one
+// partial per SpMV block and column of dotv_c . y_c, row j's term in thread
+// slot j as in the one-column kernel (the Q columns' partials reduced in one
+// pass, each over the same tree).
+template <typename T, int Q>
 __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __restrict__ blk,
                                                    const int* __restrict__ rp, const int* __restrict__ ci,
                                                    const T* __restrict__ v, const T* __restrict__ x,
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __
     constexpr int P = BParts<T>::K;
     __shared__ int s_ci[kSpmvCap];
     __shared__ T s_v[kSpmvCap];
-    __shared__ T s_term[kSpmvRows];
+    __shared__ T s_term[Q][kSpmvRows];
     const int b = blockIdx.x;
     const int r0 = blk[b], r1 = blk[b + 1];
     const int e0 = rp[r0], e1 = rp[r1];
@@ -83,47 +86,71 @@ __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __
         }
         __syncthreads();
     }
-    for (int c = 0; c < kb; ++c) {
-        const T* xc = x + (size_t)c * ld;
-        T yv = Num<T>::zero();
+    for (int c0 = 0; c0 < kb; c0 += Q) {   // kb is a multiple of Q
+        T yv[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) yv[q] = Num<T>::zero();
         if (staged) {
-            T acc = Num<T>::zero();
+            T acc[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] = Num<T>::zero();
             if (r < r1) {
                 const int e = rp[r + 1] - e0;
                 int k = rp[r] - e0 + par;
                 // gathers of a batch issued together, then summed in order
                 for (; k + 14 < e; k += 16) {
-                    T g[8];
+                    T g[8][Q];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) g[u] = xc[s_ci[k + 2 * u]];
+                    for (int u = 0; u < 8; ++u)
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = Num<T>::add(acc, Num<T>::mul(s_v[k + 2 * u], g[u]));
+                        for (int q = 0; q < Q; ++q) g[u][q] = x[(size_t)(c0 + q) * ld + s_ci[k + 2 * u]];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k + 2 * u], g[u][q]));
                 }
-                for (; k < e; k += 2) acc = Num<T>::add(acc, Num<T>::mul(s_v[k], xc[s_ci[k]]));
+                for (; k < e; k += 2) {
+                    T g[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) g[q] = x[(size_t)(c0 + q) * ld + s_ci[k]];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k], g[q]));
+                }
             }
-            const T other = shfl_xor1(acc);
-            if (r < r1 && par == 0) {
-                yv = Num<T>::add(acc, other);
-                y[(size_t)c * ld + r] = yv;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const T other = shfl_xor1(acc[q]);
+                if (r < r1 && par == 0) {
+                    yv[q] = Num<T>::add(acc[q], other);
+                    y[(size_t)(c0 + q) * ld + r] = yv[q];
+                }
             }
         } else if (threadIdx.x == 0 && r0 < r1) {   // one long row per block
-            const int e = rp[r0 + 1];
-            int k = rp[r0];
-            T ta = Num<T>::zero(), tb = Num<T>::zero();
-            while (k < e) {
-                ta = Num<T>::add(ta, Num<T>::mul(v[k], xc[ci[k]]));
-                ++k;
-                if (k < e) { tb = Num<T>::add(tb, Num<T>::mul(v[k], xc[ci[k]])); ++k; }
+            for (int q = 0; q < Q; ++q) {
+                const T* xc = x + (size_t)(c0 + q) * ld;
+                const int e = rp[r0 + 1];
+                int k = rp[r0];
+                T ta = Num<T>::zero(), tb = Num<T>::zero();
+                while (k < e) {
+                    ta = Num<T>::add(ta, Num<T>::mul(v[k], xc[ci[k]]));
+                    ++k;
+                    if (k < e) { tb = Num<T>::add(tb, Num<T>::mul(v[k], xc[ci[k]])); ++k; }
+                }
+                yv[q] = Num<T>::add(ta, tb);
+                y[(size_t)(c0 + q) * ld + r0] = yv[q];
             }
-            yv = Num<T>::add(ta, tb);
-            y[(size_t)c * ld + r0] = yv;
         }
         if (dotv) {
-            if (r < r1 && par == 0) s_term[j] = Num<T>::mul(dotv[(size_t)c * ld + r], yv);
+            if (r < r1 && par == 0)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) s_term[q][j] = Num<T>::mul(dotv[(size_t)(c0 + q) * ld + r], yv[q]);
             __syncthreads();
-            double acc[P] = {};
-            if ((int)threadIdx.x < rows) BParts<T>::put(acc, s_term[threadIdx.x]);
-            block_sum_multi<P>(acc, partial + ((size_t)b * kb + c) * P);
+            double acc[Q * P] = {};
+            if ((int)threadIdx.x < rows)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) BParts<T>::put(acc + q * P, s_term[q][threadIdx.x]);
+            // partials of columns c0 .. c0+Q-1 of this block are adjacent
+            block_sum_multi<Q * P>(acc, partial + ((size_t)b * kb + c0) * P);
             __syncthreads();
         }
     }
@@ -253,7 +280,17 @@ inline int grid_of(int64_t n, int cap) {
 template <typename T>
 void launch_spmv_cols(hipStream_t s, int kb, int ld, int nblk, const int* blk, const int* rp, const int* ci,
                       const T* v, const T* x, T* y, const T* dotv, double* partial) {
-    hipLaunchKernelGGL(k_spmv_cols<T>, dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv, partial);
+    // columns summed together: 4 real or 2 complex (kb is a power of two)
+    constexpr int QM = sizeof(T) == 8 ? 4 : 2;
+    if (kb % QM == 0)
+        hipLaunchKernelGGL((k_spmv_cols<T, QM>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
+                           partial);
+    else if (kb % 2 == 0)
+        hipLaunchKernelGGL((k_spmv_cols<T, 2>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
+                           partial);
+    else
+        hipLaunchKernelGGL((k_spmv_cols<T, 1>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
+                           partial);
 }
 template <typename T>
 void launch_dot_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, double* partial) {
