@@ -1,10 +1,13 @@
-// Kernels of block solves: kb right-hand sides at once, each vector stored
-// whole (column c at offset c * ld).  Column c of every kernel computes
-// exactly what the one-column kernel computes: the same products, the same
-// summation order per row, the same reduction trees (the same rows per
-// thread, the same wave shuffles and wave order, the block partials
-// finalized the same way), so a block solve returns, column for column, the
-// bits of the separate solves.
+// Kernels of block solves: kb right-hand sides at once, the kb columns of a
+// row side by side (entry (i, c) at i * kb + c, kb a power of two <= 32), so
+// that a gather, a poll or a store touches a row's columns in one access.
+// Column c of every kernel computes exactly what the one-column kernel
+// computes: the same products, the same summation order per row, the same
+// reduction trees (the same rows per thread, the same grid, the same wave
+// shuffles and wave order, the block partials finalized the same way), so a
+// block solve returns, column for column, the bits of the separate solves.
+// A thread of the reductions takes CQ adjacent columns of its rows
+// (workgroup column group blockIdx.y); the updates take one entry per thread.
 #include <cfloat>
 
 #include "ssgpu_kernels.h"
@@ -53,20 +56,19 @@ __device__ inline zc ntl(const zc* p) {
     return zc{__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1)};
 }
 
-// y_c = A x_c for the kb columns (column c at offset c * ld); the block's
-// entries are staged in LDS once and every column is summed over them as the
-// one-column kernel sums it (two threads per row: the even- and odd-position
-// chains, y = ta + tb).  Q columns at a time: their gathers are issued
-// together, each column's sum still in its own order.  With dotv, one
+// y = A x for the kb columns; the block's entries are staged in LDS once and
+// every column is summed over them as the one-column kernel sums it (two
+// threads per row: the even- and odd-position chains, y = ta + tb).  Q
+// adjacent columns at a time (64 bytes of an input row): their gathers are
+// issued together, each column's sum still in its own order.  With dotv, one
 // partial per SpMV block and column of dotv_c . y_c, row j's term in thread
 // slot j as in the one-column kernel (the Q columns' partials reduced in one
 // pass, each over the same tree).
 template <typename T, int Q>
-__global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __restrict__ blk,
-                                                   const int* __restrict__ rp, const int* __restrict__ ci,
-                                                   const T* __restrict__ v, const T* __restrict__ x,
-                                                   T* __restrict__ y, const T* __restrict__ dotv,
-                                                   double* __restrict__ partial) {
+__global__ __launch_bounds__(kBS) void k_spmv_il(int kb, const int* __restrict__ blk, const int* __restrict__ rp,
+                                                 const int* __restrict__ ci, const T* __restrict__ v,
+                                                 const T* __restrict__ x, T* __restrict__ y,
+                                                 const T* __restrict__ dotv, double* __restrict__ partial) {
     constexpr int P = BParts<T>::K;
     __shared__ int s_ci[kSpmvCap];
     __shared__ T s_v[kSpmvCap];
@@ -97,22 +99,27 @@ __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __
             if (r < r1) {
                 const int e = rp[r + 1] - e0;
                 int k = rp[r] - e0 + par;
-                // gathers of a batch issued together, then summed in order
-                for (; k + 14 < e; k += 16) {
-                    T g[8][Q];
+                // gathers of a batch of UB entries issued together, then
+                // summed in order
+                constexpr int UB = Q >= 8 ? 4 : 8;
+                for (; k + 2 * (UB - 1) < e; k += 2 * UB) {
+                    T g[UB][Q];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
+                    for (int u = 0; u < UB; ++u) {
+                        const T* xr = x + (size_t)s_ci[k + 2 * u] * kb + c0;
 #pragma unroll
-                        for (int q = 0; q < Q; ++q) g[u][q] = x[(size_t)(c0 + q) * ld + s_ci[k + 2 * u]];
+                        for (int q = 0; q < Q; ++q) g[u][q] = xr[q];
+                    }
 #pragma unroll
                     for (int q = 0; q < Q; ++q)
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k + 2 * u], g[u][q]));
+                        for (int u = 0; u < UB; ++u) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k + 2 * u], g[u][q]));
                 }
                 for (; k < e; k += 2) {
+                    const T* xr = x + (size_t)s_ci[k] * kb + c0;
                     T g[Q];
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) g[q] = x[(size_t)(c0 + q) * ld + s_ci[k]];
+                    for (int q = 0; q < Q; ++q) g[q] = xr[q];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k], g[q]));
                 }
@@ -122,28 +129,30 @@ __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __
                 const T other = shfl_xor1(acc[q]);
                 if (r < r1 && par == 0) {
                     yv[q] = Num<T>::add(acc[q], other);
-                    y[(size_t)(c0 + q) * ld + r] = yv[q];
+                    y[(size_t)r * kb + c0 + q] = yv[q];
                 }
             }
         } else if (threadIdx.x == 0 && r0 < r1) {   // one long row per block
             for (int q = 0; q < Q; ++q) {
-                const T* xc = x + (size_t)(c0 + q) * ld;
                 const int e = rp[r0 + 1];
                 int k = rp[r0];
                 T ta = Num<T>::zero(), tb = Num<T>::zero();
                 while (k < e) {
-                    ta = Num<T>::add(ta, Num<T>::mul(v[k], xc[ci[k]]));
+                    ta = Num<T>::add(ta, Num<T>::mul(v[k], x[(size_t)ci[k] * kb + c0 + q]));
                     ++k;
-                    if (k < e) { tb = Num<T>::add(tb, Num<T>::mul(v[k], xc[ci[k]])); ++k; }
+                    if (k < e) {
+                        tb = Num<T>::add(tb, Num<T>::mul(v[k], x[(size_t)ci[k] * kb + c0 + q]));
+                        ++k;
+                    }
                 }
                 yv[q] = Num<T>::add(ta, tb);
-                y[(size_t)(c0 + q) * ld + r0] = yv[q];
+                y[(size_t)r0 * kb + c0 + q] = yv[q];
             }
         }
         if (dotv) {
             if (r < r1 && par == 0)
 #pragma unroll
-                for (int q = 0; q < Q; ++q) s_term[q][j] = Num<T>::mul(dotv[(size_t)(c0 + q) * ld + r], yv[q]);
+                for (int q = 0; q < Q; ++q) s_term[q][j] = Num<T>::mul(dotv[(size_t)r * kb + c0 + q], yv[q]);
             __syncthreads();
             double acc[Q * P] = {};
             if ((int)threadIdx.x < rows)
@@ -156,117 +165,99 @@ __global__ __launch_bounds__(kBS) void k_spmv_cols(int kb, int ld, const int* __
     }
 }
 
-// one partial per block and column of a_c . b_c (or |a_c|^2): column c =
-// blockIdx.y, reduced exactly as the one-column k_dot / k_norm2 reduce it
-template <typename T, bool NORM>
-__global__ __launch_bounds__(kBS) void k_reduce_cols(int n, int ld, const T* __restrict__ a,
-                                                     const T* __restrict__ b, double* __restrict__ partial) {
-    constexpr int P = NORM ? 1 : BParts<T>::K;
-    const int c = blockIdx.y, kb = gridDim.y;
-    const T* ac = a + (size_t)c * ld;
-    const T* bc = NORM ? nullptr : b + (size_t)c * ld;
-    double acc[P] = {};
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        if (NORM) acc[0] += Num<T>::abs2(ac[i]);
-        else BParts<T>::put(acc, Num<T>::mul(ac[i], bc[i]));
-    }
-    block_sum_multi<P>(acc, partial + ((size_t)blockIdx.x * kb + c) * P);
-}
-
-// x_c += alpha_c p_c; r_c -= alpha_c q_c for the active columns; partials of |r_c|^2
-template <typename T>
-__global__ __launch_bounds__(kBS) void k_cg_update_cols(int n, int ld, const T* __restrict__ alpha,
-                                                        const int* __restrict__ act, const T* __restrict__ p,
-                                                        const T* __restrict__ q, T* __restrict__ x,
-                                                        T* __restrict__ r, double* __restrict__ partial) {
-    const int c = blockIdx.y, kb = gridDim.y;
-    if (!act[c]) return;   // its partial is not read
-    const T al = alpha[c];
-    const size_t o = (size_t)c * ld;
-    double acc[1] = {0.0};
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        x[o + i] = Num<T>::add(x[o + i], Num<T>::mul(al, p[o + i]));
-        const T ri = Num<T>::sub(r[o + i], Num<T>::mul(al, q[o + i]));
-        r[o + i] = ri;
-        acc[0] += Num<T>::abs2(ri);
-    }
-    block_sum_multi<1>(acc, partial + (size_t)blockIdx.x * kb + c);
-}
-
-// p_c = p_c beta_c + z_c for the active columns
-template <typename T>
-__global__ void k_cg_pupdate_cols(int n, int ld, const T* __restrict__ beta, const int* __restrict__ act,
-                                  const T* __restrict__ z, T* __restrict__ p) {
-    const int c = blockIdx.y;
-    if (!act[c]) return;
-    const T be = beta[c];
-    const size_t o = (size_t)c * ld;
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS)
-        p[o + i] = Num<T>::add(Num<T>::mul(p[o + i], be), z[o + i]);
-}
-
-// one partial per block and column of (a_c . b_c, a_c . c_c, a_c . d_c), as
-// k_dot3 forms them (the third zero without d)
-template <typename T>
-__global__ __launch_bounds__(kBS) void k_dot3_cols(int n, int ld, const T* __restrict__ a, const T* __restrict__ b,
-                                                   const T* __restrict__ cc, const T* __restrict__ d,
+// one partial per block and column of a_c . b_c (or |a_c|^2), reduced
+// exactly as the one-column k_dot / k_norm2 reduce it
+template <typename T, int CQ, bool NORM>
+__global__ __launch_bounds__(kBS) void k_reduce_il(int n, int kb, const T* __restrict__ a, const T* __restrict__ b,
                                                    double* __restrict__ partial) {
-    constexpr int P = BParts<T>::K;
-    const int c = blockIdx.y, kb = gridDim.y;
-    const size_t o = (size_t)c * ld;
-    double acc[3 * P] = {};
+    constexpr int P = NORM ? 1 : BParts<T>::K;
+    const int c0 = blockIdx.y * CQ;
+    double acc[CQ * P] = {};
     for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        const T ai = a[o + i];
-        BParts<T>::put(acc, Num<T>::mul(ai, b[o + i]));
-        BParts<T>::put(acc + P, Num<T>::mul(ai, cc[o + i]));
-        if (d) BParts<T>::put(acc + 2 * P, Num<T>::mul(ai, d[o + i]));
+        const size_t o = (size_t)i * kb + c0;
+#pragma unroll
+        for (int q = 0; q < CQ; ++q) {
+            if (NORM) acc[q] += Num<T>::abs2(a[o + q]);
+            else BParts<T>::put(acc + q * P, Num<T>::mul(a[o + q], b[o + q]));
+        }
     }
-    block_sum_multi<3 * P>(acc, partial + ((size_t)blockIdx.x * kb + c) * 3 * P);
+    block_sum_multi<CQ * P>(acc, partial + ((size_t)blockIdx.x * kb + c0) * P);
 }
 
-// MRTR updates of the active columns, column c with the scalars sc[c]
-// (t), sc[kb + c] (zeta), sc[2 kb + c] (eta).  SGS: the Eisenstat form
-// (k_sgs_update); else IC-MRTR's (k_mrtr_update, fix: keep p's recurrence).
-// Partials of |r_c|^2.
-template <typename T, bool SGS>
-__global__ __launch_bounds__(kBS) void k_mrtr_update_cols(int n, int ld, int fix, const T* __restrict__ sc,
-                                                          const int* __restrict__ act, const T* __restrict__ u,
-                                                          const T* __restrict__ v, T* __restrict__ p,
-                                                          T* __restrict__ x, T* __restrict__ y, T* __restrict__ r,
-                                                          double* __restrict__ partial) {
-    const int c = blockIdx.y, kb = gridDim.y;
-    if (!act[c]) return;   // its partial is not read
-    const T t = sc[c], zeta = sc[kb + c], eta = sc[2 * kb + c];
-    const size_t o = (size_t)c * ld;
-    double acc[1] = {0.0};
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        const T ui = u[o + i];
-        T pi;
-        if (SGS) pi = Num<T>::add(Num<T>::mul(p[o + i], t), ui);
-        else pi = fix ? Num<T>::add(ui, Num<T>::mul(t, p[o + i])) : Num<T>::add(ui, Num<T>::mul(t, ui));
-        p[o + i] = pi;
-        x[o + i] = Num<T>::add(x[o + i], Num<T>::mul(zeta, pi));
-        const T yi = Num<T>::add(Num<T>::mul(y[o + i], eta), Num<T>::mul(zeta, v[o + i]));
-        y[o + i] = yi;
-        const T ri = Num<T>::sub(r[o + i], yi);
-        r[o + i] = ri;
-        acc[0] += Num<T>::abs2(ri);
+// Updates of the active columns, one thread per entry (coalesced); column c
+// takes its scalars from sc.  The norms of the updated residuals are formed
+// afterwards by k_reduce_il, over the one-column kernels' tree.
+//   CG:   x += alpha_c p; r -= alpha_c q          (sc: alpha)
+//   CGP:  p = p beta_c + z                         (sc: beta)
+//   MRTR: p = u + t_c p (fix) or u + t_c u; x += zeta_c p; y = y eta_c + zeta_c v;
+//         r -= y                                   (sc: t, zeta, eta)
+//   SGS:  p = p t_c + u, then as MRTR              (k_sgs_update)
+//   MRTR2: z = z eta_c + zeta_c w; u -= z
+enum { UPD_CG, UPD_CGP, UPD_MRTR, UPD_SGS, UPD_MRTR2 };
+template <typename T, int OP>
+__global__ void k_update_il(int64_t total, int kb_log2, int fix, const T* __restrict__ sc,
+                            const int* __restrict__ act, const T* __restrict__ a, const T* __restrict__ b,
+                            T* __restrict__ p, T* __restrict__ x, T* __restrict__ y, T* __restrict__ r) {
+    const int kb = 1 << kb_log2;
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBS) {
+        const int c = (int)(e & (kb - 1));
+        if (!act[c]) continue;
+        if (OP == UPD_CG) {   // a = p, b = q
+            const T al = sc[c];
+            x[e] = Num<T>::add(x[e], Num<T>::mul(al, a[e]));
+            r[e] = Num<T>::sub(r[e], Num<T>::mul(al, b[e]));
+        } else if (OP == UPD_CGP) {   // a = z
+            p[e] = Num<T>::add(Num<T>::mul(p[e], sc[c]), a[e]);
+        } else if (OP == UPD_MRTR || OP == UPD_SGS) {   // a = u, b = v
+            const T t = sc[c], zeta = sc[kb + c], eta = sc[2 * kb + c];
+            const T ui = a[e];
+            T pi;
+            if (OP == UPD_SGS) pi = Num<T>::add(Num<T>::mul(p[e], t), ui);
+            else pi = fix ? Num<T>::add(ui, Num<T>::mul(t, p[e])) : Num<T>::add(ui, Num<T>::mul(t, ui));
+            p[e] = pi;
+            x[e] = Num<T>::add(x[e], Num<T>::mul(zeta, pi));
+            const T yi = Num<T>::add(Num<T>::mul(y[e], eta), Num<T>::mul(zeta, b[e]));
+            y[e] = yi;
+            r[e] = Num<T>::sub(r[e], yi);
+        } else {   // UPD_MRTR2: a = w, y = z, r = u
+            const T zi = Num<T>::add(Num<T>::mul(y[e], sc[2 * kb + c]), Num<T>::mul(sc[kb + c], a[e]));
+            y[e] = zi;
+            r[e] = Num<T>::sub(r[e], zi);
+        }
     }
-    block_sum_multi<1>(acc, partial + (size_t)blockIdx.x * kb + c);
 }
 
-// z_c = z_c eta_c + zeta_c w_c; u_c -= z_c for the active columns (k_mrtr_update2)
+// x(i, c) = x(i, c) * s_i (or / s_i), as k_mul_scale / k_div_scale per column
+template <typename T, bool DIV>
+__global__ void k_scale_il(int64_t total, int kb_log2, const T* __restrict__ sc, T* __restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBS) {
+        const T s = sc[e >> kb_log2];
+        x[e] = DIV ? Num<T>::div(x[e], s) : Num<T>::mul(x[e], s);
+    }
+}
+
+// column c of an interleaved array
 template <typename T>
-__global__ void k_mrtr_update2_cols(int n, int ld, const T* __restrict__ sc, const int* __restrict__ act,
-                                    const T* __restrict__ w, T* __restrict__ z, T* __restrict__ u) {
-    const int c = blockIdx.y, kb = gridDim.y;
-    if (!act[c]) return;
-    const T zeta = sc[kb + c], eta = sc[2 * kb + c];
-    const size_t o = (size_t)c * ld;
-    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-        const T zi = Num<T>::add(Num<T>::mul(z[o + i], eta), Num<T>::mul(zeta, w[o + i]));
-        z[o + i] = zi;
-        u[o + i] = Num<T>::sub(u[o + i], zi);
+__global__ void k_copy_col_il(int n, int kb, int c, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS)
+        dst[(size_t)i * kb + c] = src[(size_t)i * kb + c];
+}
+
+// nr vectors one after another (vector c at c * n) -> kb interleaved columns
+// (columns >= nr zero), and back
+template <typename T>
+__global__ void k_to_il(int64_t total, int n, int kb_log2, int nr, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBS) {
+        const int64_t i = e >> kb_log2;
+        const int c = (int)(e & ((1 << kb_log2) - 1));
+        dst[e] = c < nr ? src[(size_t)c * n + i] : Num<T>::zero();
+    }
+}
+template <typename T>
+__global__ void k_from_il(int64_t total, int n, int kb_log2, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)kBS + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBS) {
+        const int64_t c = e / n, i = e - c * n;
+        dst[e] = src[((size_t)i << kb_log2) + c];
     }
 }
 
@@ -274,76 +265,122 @@ inline int grid_of(int64_t n, int cap) {
     int64_t g = (n + kBS - 1) / kBS;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
+inline int log2_of(int kb) {
+    int l = 0;
+    while ((1 << l) < kb) ++l;
+    return l;
+}
+// adjacent columns per SpMV pass: 64 bytes of an input row (8 real or 4
+// complex values; fewer when kb is smaller)
+template <typename T> inline int spmv_q(int kb) { return std::min(kb, sizeof(T) == 8 ? 8 : 4); }
+// adjacent columns per thread of the reductions: 128 bytes of a row (16 real
+// or 8 complex values)
+template <typename T> inline int cq_of(int kb) { return std::min(kb, sizeof(T) == 8 ? 16 : 8); }
 
 }  // namespace
 
 template <typename T>
-void launch_spmv_cols(hipStream_t s, int kb, int ld, int nblk, const int* blk, const int* rp, const int* ci,
-                      const T* v, const T* x, T* y, const T* dotv, double* partial) {
-    // columns summed together: 4 real or 2 complex (kb is a power of two)
-    constexpr int QM = sizeof(T) == 8 ? 4 : 2;
-    if (kb % QM == 0)
-        hipLaunchKernelGGL((k_spmv_cols<T, QM>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
-                           partial);
-    else if (kb % 2 == 0)
-        hipLaunchKernelGGL((k_spmv_cols<T, 2>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
-                           partial);
+void launch_spmv_il(hipStream_t s, int kb, int nblk, const int* blk, const int* rp, const int* ci, const T* v,
+                    const T* x, T* y, const T* dotv, double* partial) {
+    const int q = spmv_q<T>(kb);
+    if (q == 8)
+        hipLaunchKernelGGL((k_spmv_il<T, 8>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
+    else if (q == 4)
+        hipLaunchKernelGGL((k_spmv_il<T, 4>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
+    else if (q == 2)
+        hipLaunchKernelGGL((k_spmv_il<T, 2>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
     else
-        hipLaunchKernelGGL((k_spmv_cols<T, 1>), dim3(nblk), dim3(kBS), 0, s, kb, ld, blk, rp, ci, v, x, y, dotv,
-                           partial);
-}
-template <typename T>
-void launch_dot_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, double* partial) {
-    hipLaunchKernelGGL((k_reduce_cols<T, false>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, a, b, partial);
-}
-template <typename T> void launch_norm2_cols(hipStream_t s, int kb, int n, int ld, const T* a, double* partial) {
-    hipLaunchKernelGGL((k_reduce_cols<T, true>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, a,
-                       (const T*)nullptr, partial);
-}
-template <typename T>
-void launch_cg_update_cols(hipStream_t s, int kb, int n, int ld, const T* alpha, const int* act, const T* p,
-                           const T* q, T* x, T* r, double* partial) {
-    hipLaunchKernelGGL(k_cg_update_cols<T>, dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, alpha, act, p, q, x,
-                       r, partial);
-}
-template <typename T>
-void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta, const int* act, const T* z, T* p) {
-    hipLaunchKernelGGL(k_cg_pupdate_cols<T>, dim3(grid_of(n, 4096), kb), dim3(kBS), 0, s, n, ld, beta, act, z, p);
+        hipLaunchKernelGGL((k_spmv_il<T, 1>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
 }
 
-template <typename T>
-void launch_dot3_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, const T* c, const T* d,
-                      double* partial) {
-    hipLaunchKernelGGL(k_dot3_cols<T>, dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, a, b, c, d, partial);
+// launch KERNEL(CQ) over (GX, kb / CQ) workgroups
+#define SSGPU_CQ(KERNEL, GX, ...)                                                                      \
+    do {                                                                                               \
+        const int cq_ = cq_of<T>(kb);                                                                  \
+        if (cq_ == 16) hipLaunchKernelGGL(KERNEL(16), dim3(GX, kb / 16), dim3(kBS), 0, s, __VA_ARGS__); \
+        else if (cq_ == 8) hipLaunchKernelGGL(KERNEL(8), dim3(GX, kb / 8), dim3(kBS), 0, s, __VA_ARGS__); \
+        else if (cq_ == 4) hipLaunchKernelGGL(KERNEL(4), dim3(GX, kb / 4), dim3(kBS), 0, s, __VA_ARGS__); \
+        else if (cq_ == 2) hipLaunchKernelGGL(KERNEL(2), dim3(GX, kb / 2), dim3(kBS), 0, s, __VA_ARGS__); \
+        else hipLaunchKernelGGL(KERNEL(1), dim3(GX, kb), dim3(kBS), 0, s, __VA_ARGS__);                \
+    } while (0)
+
+template <typename T> void launch_dot_il(hipStream_t s, int kb, int n, const T* a, const T* b, double* partial) {
+#define K_(Q) (k_reduce_il<T, Q, false>)
+    SSGPU_CQ(K_, reduce_blocks(n), n, kb, a, b, partial);
+#undef K_
+}
+template <typename T> void launch_norm2_il(hipStream_t s, int kb, int n, const T* a, double* partial) {
+#define K_(Q) (k_reduce_il<T, Q, true>)
+    SSGPU_CQ(K_, reduce_blocks(n), n, kb, a, (const T*)nullptr, partial);
+#undef K_
 }
 template <typename T>
-void launch_mrtr_update_cols(hipStream_t s, int kb, int n, int ld, bool sgs, int fix, const T* sc, const int* act,
-                             const T* u, const T* v, T* p, T* x, T* y, T* r, double* partial) {
+void launch_cg_update_il(hipStream_t s, int kb, int n, const T* alpha, const int* act, const T* p, const T* q,
+                         T* x, T* r, double* partial) {
+    const int64_t total = (int64_t)n * kb;
+    hipLaunchKernelGGL((k_update_il<T, UPD_CG>), dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total, log2_of(kb), 0,
+                       alpha, act, p, q, (T*)nullptr, x, (T*)nullptr, r);
+    launch_norm2_il<T>(s, kb, n, r, partial);
+}
+template <typename T>
+void launch_cg_pupdate_il(hipStream_t s, int kb, int n, const T* beta, const int* act, const T* z, T* p) {
+    const int64_t total = (int64_t)n * kb;
+    hipLaunchKernelGGL((k_update_il<T, UPD_CGP>), dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total, log2_of(kb), 0,
+                       beta, act, z, (const T*)nullptr, p, (T*)nullptr, (T*)nullptr, (T*)nullptr);
+}
+template <typename T>
+void launch_mrtr_update_il(hipStream_t s, int kb, int n, bool sgs, int fix, const T* sc, const int* act, const T* u,
+                           const T* v, T* p, T* x, T* y, T* r, double* partial) {
+    const int64_t total = (int64_t)n * kb;
     if (sgs)
-        hipLaunchKernelGGL((k_mrtr_update_cols<T, true>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, fix, sc,
-                           act, u, v, p, x, y, r, partial);
+        hipLaunchKernelGGL((k_update_il<T, UPD_SGS>), dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total,
+                           log2_of(kb), fix, sc, act, u, v, p, x, y, r);
     else
-        hipLaunchKernelGGL((k_mrtr_update_cols<T, false>), dim3(reduce_blocks(n), kb), dim3(kBS), 0, s, n, ld, fix,
-                           sc, act, u, v, p, x, y, r, partial);
+        hipLaunchKernelGGL((k_update_il<T, UPD_MRTR>), dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total,
+                           log2_of(kb), fix, sc, act, u, v, p, x, y, r);
+    launch_norm2_il<T>(s, kb, n, r, partial);
 }
 template <typename T>
-void launch_mrtr_update2_cols(hipStream_t s, int kb, int n, int ld, const T* sc, const int* act, const T* w, T* z,
-                              T* u) {
-    hipLaunchKernelGGL(k_mrtr_update2_cols<T>, dim3(grid_of(n, 4096), kb), dim3(kBS), 0, s, n, ld, sc, act, w, z, u);
+void launch_mrtr_update2_il(hipStream_t s, int kb, int n, const T* sc, const int* act, const T* w, T* z, T* u) {
+    const int64_t total = (int64_t)n * kb;
+    hipLaunchKernelGGL((k_update_il<T, UPD_MRTR2>), dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total, log2_of(kb),
+                       0, sc, act, w, (const T*)nullptr, (T*)nullptr, (T*)nullptr, z, u);
+}
+#undef SSGPU_CQ
+
+template <typename T> void launch_scale_il(hipStream_t s, int kb, int n, const T* sc, T* x, bool div) {
+    const int64_t total = (int64_t)n * kb;
+    const int g = grid_of(total, 8192);
+    if (div) hipLaunchKernelGGL((k_scale_il<T, true>), dim3(g), dim3(kBS), 0, s, total, log2_of(kb), sc, x);
+    else hipLaunchKernelGGL((k_scale_il<T, false>), dim3(g), dim3(kBS), 0, s, total, log2_of(kb), sc, x);
+}
+template <typename T> void launch_copy_col_il(hipStream_t s, int kb, int n, int c, const T* src, T* dst) {
+    hipLaunchKernelGGL(k_copy_col_il<T>, dim3(grid_of(n, 4096)), dim3(kBS), 0, s, n, kb, c, src, dst);
+}
+template <typename T> void launch_to_il(hipStream_t s, int kb, int n, int nr, const T* src, T* dst) {
+    const int64_t total = (int64_t)n * kb;
+    hipLaunchKernelGGL(k_to_il<T>, dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total, n, log2_of(kb), nr, src, dst);
+}
+template <typename T> void launch_from_il(hipStream_t s, int kb, int n, int nr, const T* src, T* dst) {
+    const int64_t total = (int64_t)n * nr;
+    hipLaunchKernelGGL(k_from_il<T>, dim3(grid_of(total, 8192)), dim3(kBS), 0, s, total, n, log2_of(kb), src, dst);
 }
 
 #define SSGPU_INST(T)                                                                                              \
-    template void launch_spmv_cols<T>(hipStream_t, int, int, int, const int*, const int*, const int*, const T*,     \
-                                      const T*, T*, const T*, double*);                                         \
-    template void launch_dot_cols<T>(hipStream_t, int, int, int, const T*, const T*, double*);                    \
-    template void launch_norm2_cols<T>(hipStream_t, int, int, int, const T*, double*);                            \
-    template void launch_cg_update_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, const T*, T*,  \
-                                           T*, double*);                                                        \
-    template void launch_cg_pupdate_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, T*);      \
-    template void launch_dot3_cols<T>(hipStream_t, int, int, int, const T*, const T*, const T*, const T*, double*); \
-    template void launch_mrtr_update_cols<T>(hipStream_t, int, int, int, bool, int, const T*, const int*, const T*, \
-                                             const T*, T*, T*, T*, T*, double*);                                \
-    template void launch_mrtr_update2_cols<T>(hipStream_t, int, int, int, const T*, const int*, const T*, T*, T*);
+    template void launch_spmv_il<T>(hipStream_t, int, int, const int*, const int*, const int*, const T*, const T*, \
+                                    T*, const T*, double*);                                                       \
+    template void launch_dot_il<T>(hipStream_t, int, int, const T*, const T*, double*);                           \
+    template void launch_norm2_il<T>(hipStream_t, int, int, const T*, double*);                                   \
+    template void launch_cg_update_il<T>(hipStream_t, int, int, const T*, const int*, const T*, const T*, T*, T*, \
+                                         double*);                                                                \
+    template void launch_cg_pupdate_il<T>(hipStream_t, int, int, const T*, const int*, const T*, T*);             \
+    template void launch_mrtr_update_il<T>(hipStream_t, int, int, bool, int, const T*, const int*, const T*,     \
+                                           const T*, T*, T*, T*, T*, double*);                                    \
+    template void launch_mrtr_update2_il<T>(hipStream_t, int, int, const T*, const int*, const T*, T*, T*);       \
+    template void launch_scale_il<T>(hipStream_t, int, int, const T*, T*, bool);                                  \
+    template void launch_copy_col_il<T>(hipStream_t, int, int, int, const T*, T*);                                \
+    template void launch_to_il<T>(hipStream_t, int, int, int, const T*, T*);                                      \
+    template void launch_from_il<T>(hipStream_t, int, int, int, const T*, T*);
 SSGPU_INST(double)
 SSGPU_INST(zc)
 #undef SSGPU_INST

@@ -325,7 +325,6 @@ static void analyse(Matrix& m) {
     upload(c, m.blk, blk);
     if (const char* g = std::getenv("SSGPU_SWEEP_GRID")) m.sweep_grid = std::max(1, std::atoi(g));
     if (const char* g = std::getenv("SSGPU_SWEEP_IDLE")) m.sweep_idle = std::min(5, std::max(1, std::atoi(g)));
-    if (const char* g = std::getenv("SSGPU_BLOCK_KC")) m.block_kc = std::atoi(g);
     if (const char* g = std::getenv("SSGPU_BLOCK_GRID")) m.block_grid = std::max(1, std::atoi(g));
     std::vector<int> diag(n, -1);
     parallel_for(n, [&](int i) {
@@ -1345,16 +1344,11 @@ template <typename T> struct BlockSolve {
     Matrix& m;
     hipStream_t s;
     int n, kb, P;
-    int kc;   // columns per lane of the block sweeps
     using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
-    explicit BlockSolve(Solver& SS, int kbb) : S(SS), c(SS.c), m(SS.m), s(SS.s), n(SS.m.n), kb(kbb), P(kparts<T>()) {
-        // one column per lane measured best (shortest polling pass; the
-        // columns' dependency chains run side by side in separate groups)
-        const int lim = sizeof(T) == 8 ? 4 : 2;
-        kc = m.block_kc > 0 ? m.block_kc : 1;
-        if (kc > lim || (kc != 1 && kc != 2 && kc != 4)) kc = 1;
-        while (kc > 1 && kb % kc) kc /= 2;
-    }
+    explicit BlockSolve(Solver& SS, int kbb) : S(SS), c(SS.c), m(SS.m), s(SS.s), n(SS.m.n), kb(kbb), P(kparts<T>()) {}
+    // polling waves of a block sweep: 128 per column up to 2048 in all
+    // (measured best at n=95 for 8, 16 and 32 columns)
+    int sweep_waves() const { return m.block_grid > 0 ? m.block_grid : std::min(2048, 128 * kb); }
 
     T* bvec(size_t idx) {   // n x kb work array from the context's pool
         while (c->work.size() <= idx) c->work.emplace_back(new DevBuf());
@@ -1365,7 +1359,7 @@ template <typename T> struct BlockSolve {
     }
     double* bpartial() {
         // per SpMV block at most kb * P = 64 dot parts; per reduction block
-        // at most 3 kb P = 192 (the MRTR three-dot phase)
+        // at most three regions of kb * P (the MRTR phase's three dots)
         const size_t need = sizeof(double) * std::max(64 * (size_t)m.nblk, 192 * (size_t)kMaxReduceBlocks);
         if (c->d_bpartial.bytes < need) c->d_bpartial.alloc(c, need);
         return c->d_bpartial.as<double>();
@@ -1408,30 +1402,28 @@ template <typename T> struct BlockSolve {
     const T* d_scalars() { return c->d_bscal.as<T>(); }
     const int* d_act() { return reinterpret_cast<const int*>(static_cast<char*>(c->d_bscal.p) + kActOff); }
 
+    // z = M^-1 r for every column (the interleaved block sweeps)
     void ic_apply(const T* r, T* z, T* tmp) {
         launch_sf_prepare<T>(s, n * kb, tmp, z, S.tickets());
         int* tk = btickets();
         launch_fill<int>(s, 64, tk, 0);
-        const int ng = kb / kc;
-        // about 512 polling waves in all (fewer per group as groups multiply)
-        const int grid = m.block_grid > 0 ? m.block_grid : std::min(m.sweep_grid, std::max(32, 512 / ng));
-        const bool f32 = S.o.factor_precision == SSGPU_FACTOR_F32;
-        if (f32) {
-            launch_sweep_block<T, F>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit,
-                                     m.sfL.srow, m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
-                                     m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r, nullptr, nullptr, tmp);
-            launch_sweep_block<T, F>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(), S.spin_limit,
-                                     m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
-                                     m.sfUic.ecol.as<int>(), m.sfUic.eval32.as<F>(), nullptr, nullptr, tmp,
-                                     m.diagD.as<T>(), z);
+        const int grid = sweep_waves();
+        if (S.o.factor_precision == SSGPU_FACTOR_F32) {
+            launch_sweep_il<T, F>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, tk, S.errflag(), S.spin_limit, m.sfL.srow,
+                                  m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+                                  m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r, nullptr, nullptr, tmp);
+            launch_sweep_il<T, F>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, tk + 1, S.errflag(), S.spin_limit,
+                                  m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
+                                  m.sfUic.ecol.as<int>(), m.sfUic.eval32.as<F>(), nullptr, nullptr, tmp,
+                                  m.diagD.as<T>(), z);
         } else {
-            launch_sweep_block<T, T>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit,
-                                     m.sfL.srow, m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
-                                     m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr, nullptr, tmp);
-            launch_sweep_block<T, T>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, kc, tk + ng, S.errflag(), S.spin_limit,
-                                     m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
-                                     m.sfUic.ecol.as<int>(), m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
-                                     m.diagD.as<T>(), z);
+            launch_sweep_il<T, T>(s, FWD_DIV_LAST, m.sfL.kmax, n, grid, kb, tk, S.errflag(), S.spin_limit, m.sfL.srow,
+                                  m.sfL.slice_off.as<int>(), m.sfL.rlen.as<int>(), m.sfL.ecol.as<int>(),
+                                  m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r, nullptr, nullptr, tmp);
+            launch_sweep_il<T, T>(s, BWD_IC, m.sfUic.kmax, n, grid, kb, tk + 1, S.errflag(), S.spin_limit,
+                                  m.sfUic.srow, m.sfUic.slice_off.as<int>(), m.sfUic.rlen.as<int>(),
+                                  m.sfUic.ecol.as<int>(), m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
+                                  m.diagD.as<T>(), z);
         }
     }
 
@@ -1441,18 +1433,17 @@ template <typename T> struct BlockSolve {
         launch_sf_prepare<T>(s, n * kb, z, nullptr, S.tickets());
         int* tk = btickets();
         launch_fill<int>(s, 64, tk, 0);
-        const int ng = kb / kc;
-        const int grid = m.block_grid > 0 ? m.block_grid : std::min(m.sweep_grid, std::max(32, 512 / ng));
+        const int grid = sweep_waves();
         const SfCsr& sf = forward ? m.sfL : m.sfUs;
         const int mode = forward ? FWD_DIV_LAST : BWD_DIV_FIRST;
         if (S.o.factor_precision == SSGPU_FACTOR_F32)
-            launch_sweep_block<T, F>(s, mode, sf.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit, sf.srow,
-                                     sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval32.as<F>(),
-                                     sf.dval32.as<F>(), r, nullptr, nullptr, z);
+            launch_sweep_il<T, F>(s, mode, sf.kmax, n, grid, kb, tk, S.errflag(), S.spin_limit, sf.srow,
+                                  sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval32.as<F>(),
+                                  sf.dval32.as<F>(), r, nullptr, nullptr, z);
         else
-            launch_sweep_block<T, T>(s, mode, sf.kmax, n, grid, kb, kc, tk, S.errflag(), S.spin_limit, sf.srow,
-                                     sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval.as<T>(),
-                                     sf.dval.as<T>(), r, nullptr, nullptr, z);
+            launch_sweep_il<T, T>(s, mode, sf.kmax, n, grid, kb, tk, S.errflag(), S.spin_limit, sf.srow,
+                                  sf.slice_off.as<int>(), sf.rlen.as<int>(), sf.ecol.as<int>(), sf.eval.as<T>(),
+                                  sf.dval.as<T>(), r, nullptr, nullptr, z);
     }
 
     struct Col {
@@ -1473,14 +1464,14 @@ template <typename T> struct BlockSolve {
         if (o.is_init) launch_fill<T>(s, nk, X, Host<T>::zero());
         if (BEST) launch_fill<T>(s, nk, BEST, Host<T>::zero());
         if (!o.is_init) {
-            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), X,
+            launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), X,
                                 AP, nullptr, nullptr);
             launch_residual<T>(s, (int)nk, B, AP, R);
         } else {
             launch_residual<T>(s, (int)nk, B, nullptr, R);
         }
         bpartial();
-        launch_norm2_cols<T>(s, kb, n, n, R, c->d_bpartial.as<double>());
+        launch_norm2_il<T>(s, kb, n, R, c->d_bpartial.as<double>());
         {
             const int K = kb;   // norms: one part per column
             launch_finalize(s, reduce_blocks(n), K, c->d_bpartial.as<double>(), c->d_scal.as<double>());
@@ -1506,12 +1497,12 @@ template <typename T> struct BlockSolve {
         if (!any_active()) return col;
         ic_apply(R, P_, TMP);
         launch_copy<T>(s, nk, P_, Z);
-        launch_dot_cols<T>(s, kb, n, n, R, Z, c->d_bpartial.as<double>());
+        launch_dot_il<T>(s, kb, n, R, Z, c->d_bpartial.as<double>());
         std::vector<T> rur0 = dots(fetch(reduce_blocks(n)));
         std::vector<T> sc(kb, Host<T>::zero());
         std::vector<int> act(kb, 0);
         for (int it = 0; it < max_ite && any_active(); ++it) {
-            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(),
+            launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(),
                                 P_, AP, P_, c->d_bpartial.as<double>());
             const std::vector<T> t2 = dots(fetch(m.nblk));
             for (int j = 0; j < kb; ++j) {
@@ -1519,7 +1510,7 @@ template <typename T> struct BlockSolve {
                 sc[j] = col[j].active ? Host<T>::div(rur0[j], t2[j]) : Host<T>::zero();
             }
             put_scalars(sc, act);
-            launch_cg_update_cols<T>(s, kb, n, n, d_scalars(), d_act(), P_, AP, X, R, c->d_bpartial.as<double>());
+            launch_cg_update_il<T>(s, kb, n, d_scalars(), d_act(), P_, AP, X, R, c->d_bpartial.as<double>());
             std::vector<double> pn;
             {
                 launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
@@ -1536,7 +1527,7 @@ template <typename T> struct BlockSolve {
                 if (normR < conv_cri || p < q.abs_conv) { q.active = false; q.conv = true; continue; }
                 if (normR < q.best) {
                     q.best = normR;
-                    if (BEST) launch_copy<T>(s, n, X + (size_t)j * n, BEST + (size_t)j * n);
+                    if (BEST) launch_copy_col_il<T>(s, kb, n, j, X, BEST);
                 }
                 if (o.diverge_judge_type == 1) {
                     if (normR < q.best * o.bad_div_val) q.bad = 0;
@@ -1546,7 +1537,7 @@ template <typename T> struct BlockSolve {
             }
             if (!any_active()) break;
             ic_apply(R, Z, TMP);
-            launch_dot_cols<T>(s, kb, n, n, R, Z, c->d_bpartial.as<double>());
+            launch_dot_il<T>(s, kb, n, R, Z, c->d_bpartial.as<double>());
             const std::vector<T> rur1 = dots(fetch(reduce_blocks(n)));
             for (int j = 0; j < kb; ++j) {
                 act[j] = col[j].active ? 1 : 0;
@@ -1554,11 +1545,11 @@ template <typename T> struct BlockSolve {
                 if (col[j].active) rur0[j] = rur1[j];
             }
             put_scalars(sc, act);
-            launch_cg_pupdate_cols<T>(s, kb, n, n, d_scalars(), d_act(), Z, P_);
+            launch_cg_pupdate_il<T>(s, kb, n, d_scalars(), d_act(), Z, P_);
         }
         if (BEST)
             for (int j = 0; j < kb; ++j)
-                if (!col[j].conv) launch_copy<T>(s, n, BEST + (size_t)j * n, X + (size_t)j * n);
+                if (!col[j].conv) launch_copy_col_il<T>(s, kb, n, j, BEST, X);
         return col;
     }
 
@@ -1612,14 +1603,14 @@ template <typename T> struct BlockSolve {
         // MatSolverMRTR_SGSver.cpp:112-114 subtract A x only when is_init
         T* r0 = sgs ? TMP : R;
         if (o.is_init) {
-            launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, X, V, nullptr,
+            launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, X, V, nullptr,
                                 nullptr);
             launch_residual<T>(s, (int)nk, B, V, r0);
         } else {
             launch_residual<T>(s, (int)nk, B, nullptr, r0);
         }
         if (sgs) sgs_sweep(true, TMP, R);   // Rd = fr(b - A x)
-        launch_norm2_cols<T>(s, kb, n, n, R, c->d_bpartial.as<double>());
+        launch_norm2_il<T>(s, kb, n, R, c->d_bpartial.as<double>());
         if (!prologue(col, conv_cri, normB, fetch_norms(reduce_blocks(n)))) return col;
         if (!sgs) ic_apply(R, U, TMP);
         launch_negate<T>(s, (int)nk, R, Y);
@@ -1639,7 +1630,7 @@ template <typename T> struct BlockSolve {
             if (sgs) {
                 sgs_sweep(false, R, U);   // U = bc(Rd)
                 if (f32) {
-                    launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U,
+                    launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U,
                                         TMP, nullptr, nullptr);
                     sgs_sweep(true, TMP, V);
                 } else {
@@ -1650,23 +1641,34 @@ template <typename T> struct BlockSolve {
                 dv = V;
                 dw = V;
             } else {
-                launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U, V,
+                launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), av, U, V,
                                     nullptr, nullptr);
                 ic_apply(V, W, TMP);
             }
-            launch_dot3_cols<T>(s, kb, n, n, dw, R, dv, it == 0 ? nullptr : Y, c->d_bpartial.as<double>());
-            const std::vector<double> d3 = fetch(reduce_blocks(n), 3);
+            // the three dots of k_dot3 (w.r, w.v and, from the second
+            // iteration on, w.y), each over its own partials and finalized
+            // into its own third of the scalars (the same trees per value)
+            {
+                const int nb = reduce_blocks(n);
+                double* part = c->d_bpartial.as<double>();
+                double* out = c->d_scal.as<double>();
+                const size_t reg = (size_t)nb * kb * P;
+                const int nd = it == 0 ? 2 : 3;
+                const T* rhs3[3] = {R, dv, Y};
+                for (int k = 0; k < nd; ++k) launch_dot_il<T>(s, kb, n, dw, rhs3[k], part + k * reg);
+                for (int k = 0; k < nd; ++k) launch_finalize(s, nb, kb * P, part + k * reg, out + (size_t)k * kb * P);
+            }
+            const double* d3 = read_scalars(c);
             for (int j = 0; j < kb; ++j) {
                 act[j] = col[j].active ? 1 : 0;
                 if (!col[j].active) continue;
-                const double* d = &d3[(size_t)j * 3 * P];
-                const T w_r = Host<T>::from(d), v_w = Host<T>::from(d + P);
+                const T w_r = Host<T>::from(d3 + (size_t)j * P), v_w = Host<T>::from(d3 + (size_t)(kb + j) * P);
                 if (it == 0) {
                     zeta[j] = Host<T>::div(w_r, v_w);
                     zeta_old[j] = zeta[j];
                     eta[j] = Host<T>::zero();
                 } else {
-                    const T w_y = Host<T>::from(d + 2 * P);
+                    const T w_y = Host<T>::from(d3 + (size_t)(2 * kb + j) * P);
                     const T t0 = hdiv(1.0, nu[j] * v_w - w_y * w_y);
                     zeta[j] = nu[j] * w_r * t0;
                     eta[j] = -1.0 * w_y * w_r * t0;
@@ -1678,7 +1680,7 @@ template <typename T> struct BlockSolve {
                 zeta_old[j] = zeta[j];
             }
             put_scalars(sc, act);
-            launch_mrtr_update_cols<T>(s, kb, n, n, sgs, o.mrtr_direction_fix, d_scalars(), d_act(), U, dv, P_, Xi, Y,
+            launch_mrtr_update_il<T>(s, kb, n, sgs, o.mrtr_direction_fix, d_scalars(), d_act(), U, dv, P_, Xi, Y,
                                        R, c->d_bpartial.as<double>());
             const std::vector<double> pn = fetch_norms(reduce_blocks(n));
             for (int j = 0; j < kb; ++j) {   // Solver::monitor, per column
@@ -1691,7 +1693,7 @@ template <typename T> struct BlockSolve {
                 if (normR < conv_cri || p < q.abs_conv) { q.active = false; q.conv = true; continue; }
                 if (normR < q.best) {
                     q.best = normR;
-                    if (BEST) launch_copy<T>(s, n, Xi + (size_t)j * n, BEST + (size_t)j * n);
+                    if (BEST) launch_copy_col_il<T>(s, kb, n, j, Xi, BEST);
                 }
                 if (o.diverge_judge_type == 1) {
                     if (normR < q.best * o.bad_div_val) q.bad = 0;
@@ -1703,13 +1705,13 @@ template <typename T> struct BlockSolve {
             if (!sgs) {
                 for (int j = 0; j < kb; ++j) act[j] = col[j].active ? 1 : 0;
                 put_scalars(sc, act);
-                launch_mrtr_update2_cols<T>(s, kb, n, n, d_scalars(), d_act(), W, Zv, U);
+                launch_mrtr_update2_il<T>(s, kb, n, d_scalars(), d_act(), W, Zv, U);
             }
         }
         launch_copy<T>(s, nk, Xi, X);
         if (BEST)
             for (int j = 0; j < kb; ++j)
-                if (!col[j].conv) launch_copy<T>(s, n, BEST + (size_t)j * n, X + (size_t)j * n);
+                if (!col[j].conv) launch_copy_col_il<T>(s, kb, n, j, BEST, X);
         return col;
     }
 };
@@ -1749,15 +1751,12 @@ int solve_block(Context* c, Matrix& m, int solver, const ssgpu_options& o, doubl
     SSGPU_HIP(hipEventCreate(&e1));
     SSGPU_HIP(hipEventCreate(&e2));
     SSGPU_HIP(hipEventRecord(e1, s));
+    // the caller's vectors (one after another) to the block layout, the kb
+    // columns of a row side by side (padding columns zero)
     T* Bp = B.bvec(26);
     T* Xp = B.bvec(27);
-    const int64_t nr = (int64_t)n * nrhs, npad = (int64_t)n * (kb - nrhs);
-    launch_copy<T>(s, nr, b, Bp);
-    launch_copy<T>(s, nr, x, Xp);
-    if (npad) {
-        launch_fill<T>(s, npad, Bp + nr, Host<T>::zero());
-        launch_fill<T>(s, npad, Xp + nr, Host<T>::zero());
-    }
+    launch_to_il<T>(s, kb, n, nrhs, b, Bp);
+    launch_to_il<T>(s, kb, n, nrhs, x, Xp);
     std::vector<double> nB(kb, 0.0);
     std::vector<typename BlockSolve<T>::Col> col;
     if (sgs) {
@@ -1767,31 +1766,29 @@ int solve_block(Context* c, Matrix& m, int solver, const ssgpu_options& o, doubl
         T* Xs = B.bvec(33);
         launch_copy<T>(s, (int64_t)n * kb, Bp, Bs);
         launch_copy<T>(s, (int64_t)n * kb, Xp, Xs);
-        for (int j = 0; j < kb; ++j) {
-            launch_mul_scale<T>(s, n, m.sc.as<T>(), Bs + (size_t)j * n);
-            if (!o.is_init) launch_div_scale<T>(s, n, m.sc.as<T>(), Xs + (size_t)j * n);
-        }
+        launch_scale_il<T>(s, kb, n, m.sc.as<T>(), Bs, false);
+        if (!o.is_init) launch_scale_il<T>(s, kb, n, m.sc.as<T>(), Xs, true);
         B.ensure_scal();
         B.bpartial();
-        launch_norm2_cols<T>(s, kb, n, n, Bs, c->d_bpartial.as<double>());
+        launch_norm2_il<T>(s, kb, n, Bs, c->d_bpartial.as<double>());
         const std::vector<double> n2 = B.fetch_norms(reduce_blocks(n));
         for (int j = 0; j < kb; ++j) nB[j] = std::sqrt(n2[j]);
         col = B.run_mrtr(true, conv_cri, max_ite, nB, Bs, Xs);
-        for (int j = 0; j < kb; ++j) launch_mul_scale<T>(s, n, m.sc.as<T>(), Xs + (size_t)j * n);
+        launch_scale_il<T>(s, kb, n, m.sc.as<T>(), Xs, false);
         launch_copy<T>(s, (int64_t)n * kb, Xs, Xp);
     } else {
         for (int j = 0; j < nrhs; ++j) nB[j] = normB[j];
         col = solver == SSGPU_ICCG ? B.run(conv_cri, max_ite, nB, Bp, Xp)
                                    : B.run_mrtr(false, conv_cri, max_ite, nB, Bp, Xp);
     }
-    launch_copy<T>(s, nr, Xp, x);
+    launch_from_il<T>(s, kb, n, nrhs, Xp, x);
     // true residuals ||b_j - A x_j|| / normB_j
     T* Rt = B.bvec(21);
-    launch_spmv_cols<T>(s, kb, n, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), Xp, Rt,
+    launch_spmv_il<T>(s, kb, m.nblk, m.blk.as<int>(), m.rp.as<int>(), m.ci.as<int>(), m.val.as<T>(), Xp, Rt,
                         nullptr, nullptr);
     launch_residual<T>(s, n * kb, Bp, Rt, Rt);
     B.bpartial();
-    launch_norm2_cols<T>(s, kb, n, n, Rt, c->d_bpartial.as<double>());
+    launch_norm2_il<T>(s, kb, n, Rt, c->d_bpartial.as<double>());
     launch_finalize(s, reduce_blocks(n), kb, c->d_bpartial.as<double>(), c->d_scal.as<double>());
     SSGPU_HIP(hipMemcpyAsync(c->h_scal, c->d_scal.p, sizeof(double) * kb, hipMemcpyDeviceToHost, s));
     SSGPU_HIP(hipEventRecord(e2, s));

@@ -179,8 +179,7 @@ struct Matrix {
     bool sweeps_tuned = false;   // sweep launch shape measured (first factorization)
     int spmv_stage_kb = 16;      // SpMV staging (16 or 32 KB), measured at creation
     int sweep_idle = 2;       // longest back-off step of a waiting wave (2: 128 clocks ... 5: 1024)
-    int block_kc = 0;         // block sweeps: columns per lane (0: 1)
-    int block_grid = 0;       // block sweeps: workgroups per column group (0: about 512 waves in all)
+    int block_grid = 0;       // block sweeps: polling waves in all (0: 128 per column, at most 2048)
     bool uic_built = false, us_built = false;
     // factor / preconditioner values and work space (grown on demand)
     DevBuf Lv, Uv, Lv32, Uv32, diagD, scaledA, sc, bs;

@@ -174,38 +174,38 @@ template <typename T, typename F>
 void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int* levptr, const int* srow,
                          const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                          const T* rhs, const T* aux, const T* dd, T* x);
-// block sweep over ncol right-hand sides (column c at offset c * n), kc in
-// {1, 2, 4 (real only)} columns per lane (registers: KMAX x kc inputs),
-// ncol a multiple of kc; tickets: ncol / kc zeroed counters
+// block sweep over kb right-hand sides with entry (i, c) at i * kb + c (kb a
+// power of two <= 32; rhs, aux and x in that layout); grid: waves in all;
+// ticket: one zeroed counter
 template <typename T, typename F>
-void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
-                        long spin_limit,
-                        const int* srow, const int* slice_off, const int* rlen, const int* ecol, const F* eval,
-                        const F* dval, const T* rhs, const T* aux, const T* dd, T* x);
-// --- block solves (ssgpu_block.hip): kb columns, column c at offset c * ld ------------
+void launch_sweep_il(hipStream_t s, int mode, int kmax, int n, int grid, int kb, int* ticket, int* err,
+                     long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                     const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x);
+// --- block solves (ssgpu_block.hip): kb columns side by side, entry (i, c) at i * kb + c ----
+// (kb a power of two <= 32); every column computed as the one-column kernel computes it
 template <typename T>
-void launch_spmv_cols(hipStream_t s, int kb, int ld, int nblk, const int* blk, const int* rp, const int* ci,
-                      const T* v, const T* x, T* y, const T* dotv, double* partial);
+void launch_spmv_il(hipStream_t s, int kb, int nblk, const int* blk, const int* rp, const int* ci, const T* v,
+                    const T* x, T* y, const T* dotv, double* partial);
 // partials: kb * parts per block ((block * kb + c) * parts), finalized by launch_finalize
+template <typename T> void launch_dot_il(hipStream_t s, int kb, int n, const T* a, const T* b, double* partial);
+template <typename T> void launch_norm2_il(hipStream_t s, int kb, int n, const T* a, double* partial);
 template <typename T>
-void launch_dot_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, double* partial);
-template <typename T> void launch_norm2_cols(hipStream_t s, int kb, int n, int ld, const T* a, double* partial);
+void launch_cg_update_il(hipStream_t s, int kb, int n, const T* alpha, const int* act, const T* p, const T* q,
+                         T* x, T* r, double* partial);
 template <typename T>
-void launch_cg_update_cols(hipStream_t s, int kb, int n, int ld, const T* alpha, const int* act, const T* p,
-                           const T* q, T* x, T* r, double* partial);
-template <typename T>
-void launch_cg_pupdate_cols(hipStream_t s, int kb, int n, int ld, const T* beta, const int* act, const T* z, T* p);
-// partials: kb * 3 * parts per block, as launch_dot3 per column (d may be null)
-template <typename T>
-void launch_dot3_cols(hipStream_t s, int kb, int n, int ld, const T* a, const T* b, const T* c, const T* d,
-                      double* partial);
+void launch_cg_pupdate_il(hipStream_t s, int kb, int n, const T* beta, const int* act, const T* z, T* p);
 // sc: t[kb], zeta[kb], eta[kb]; sgs: the SGS-MRTR update, else IC-MRTR's
 template <typename T>
-void launch_mrtr_update_cols(hipStream_t s, int kb, int n, int ld, bool sgs, int fix, const T* sc, const int* act,
-                             const T* u, const T* v, T* p, T* x, T* y, T* r, double* partial);
+void launch_mrtr_update_il(hipStream_t s, int kb, int n, bool sgs, int fix, const T* sc, const int* act, const T* u,
+                           const T* v, T* p, T* x, T* y, T* r, double* partial);
 template <typename T>
-void launch_mrtr_update2_cols(hipStream_t s, int kb, int n, int ld, const T* sc, const int* act, const T* w, T* z,
-                              T* u);
+void launch_mrtr_update2_il(hipStream_t s, int kb, int n, const T* sc, const int* act, const T* w, T* z, T* u);
+// x(i, c) *= s_i (div: /= s_i)
+template <typename T> void launch_scale_il(hipStream_t s, int kb, int n, const T* sc, T* x, bool div);
+template <typename T> void launch_copy_col_il(hipStream_t s, int kb, int n, int c, const T* src, T* dst);
+// nr vectors one after another <-> kb interleaved columns (columns >= nr zero)
+template <typename T> void launch_to_il(hipStream_t s, int kb, int n, int nr, const T* src, T* dst);
+template <typename T> void launch_from_il(hipStream_t s, int kb, int n, int nr, const T* src, T* dst);
 // --- batched solves (many independent systems, one launch per phase) ----------------
 // Each system keeps its own vectors and scalar state; a launch covers every
 // system of the batch whose status equals the launch's `want` (the others are

@@ -462,112 +462,6 @@ __global__ __launch_bounds__(kSfRows * WPB) void k_sweep_lvl(int p0, int p1, con
     x[row.i] = row.result(g);
 }
 
-// Several right-hand sides at once (block solves): x, rhs and aux hold ncol
-// vectors of n entries (entry (i, c) at c * n + i).  Workgroup (., g) sweeps
-// columns [g*KC, g*KC + KC) with its own ticket counter tickets[g]; a lane
-// keeps its row's entries once and KC inputs per entry.  Each column's
-// result is formed exactly as the one-column sweep forms it.
-template <typename T, typename F, int MODE, int KMAX, int KC>
-__global__ __launch_bounds__(kSfRows) void k_sweep_block(int n, int nchunks, int ncol, int* __restrict__ tickets,
-                                                         int* __restrict__ err, long spin_limit,
-                                                         const int* __restrict__ srow,
-                                                         const int* __restrict__ slice_off,
-                                                         const int* __restrict__ rlen, const int* __restrict__ ecol,
-                                                         const F* __restrict__ eval, const F* __restrict__ dval,
-                                                         const T* __restrict__ rhs, const T* __restrict__ aux,
-                                                         const T* __restrict__ dd, T* __restrict__ x) {
-    const int lane = threadIdx.x;
-    const int c0 = (int)blockIdx.y * KC;
-    int* ticket = tickets + blockIdx.y;
-    long spins = 0;
-    while (true) {
-        const int c = take_ticket(ticket);
-        if (c >= nchunks) return;
-        const int p = c * kSfRows + lane;
-        const bool active = p < n;
-        const int i = active ? srow[p] : 0;
-        const int len = active ? rlen[p] : 0;
-        unsigned cols[KMAX];
-        F vals[KMAX];
-        const int base = active ? slice_off[c] + lane : 0;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {   // branch-free, as in RowState::load
-            const int at = k < len ? base + kSfRows * k : 0;
-            cols[k] = (unsigned)ecol[at];
-            vals[k] = eval[at];
-        }
-        T rv[KC];
-        T d_i = Num<T>::zero();
-        F dg = F();
-#pragma unroll
-        for (int q = 0; q < KC; ++q) rv[q] = Num<T>::zero();
-        if (active) {
-            if (MODE == BWD_IC) {
-                d_i = dd[i];
-#pragma unroll
-                for (int q = 0; q < KC; ++q) rv[q] = aux[(size_t)(c0 + q) * n + i];
-            } else {
-                dg = dval[p];
-#pragma unroll
-                for (int q = 0; q < KC; ++q) rv[q] = rhs[(size_t)(c0 + q) * n + i];
-            }
-        }
-        T g[KMAX][KC];
-        unsigned miss = active ? (len >= 32 ? ~0u : ((1u << len) - 1u)) : 0u;
-        bool done = !active;
-        int idle = 0;
-        while (true) {
-            __asm__ __volatile__("" ::: "memory");
-            bool progress = false;
-            if (!done && miss) {
-                const unsigned want = miss;
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (want & (1u << k)) {
-#pragma unroll
-                        for (int q = 0; q < KC; ++q) g[k][q] = gload(x + (size_t)(c0 + q) * n + cols[k]);
-                    }
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k)
-                    if (want & (1u << k)) {
-                        bool ok = true;
-#pragma unroll
-                        for (int q = 0; q < KC; ++q) ok = ok && !sent(g[k][q]);
-                        if (ok) miss &= ~(1u << k);
-                    }
-                progress = miss != want;
-            }
-            if (!done && miss == 0u) {
-#pragma unroll
-                for (int q = 0; q < KC; ++q) {
-                    T sum = (MODE == BWD_IC) ? Num<T>::zero() : rv[q];
-#pragma unroll
-                    for (int k = 0; k < KMAX; ++k)
-                        if (k < len) sum = Num<T>::sub(sum, Num<T>::mul(wid<T, F>(vals[k]), g[k][q]));
-                    const T out = (MODE == BWD_IC) ? Num<T>::add(Num<T>::mul(sum, d_i), rv[q])
-                                                   : Num<T>::div(sum, wid<T, F>(dg));
-                    gstore(x + (size_t)(c0 + q) * n + i, out);
-                }
-                done = true;
-                progress = true;
-            }
-            if (__all(done)) break;
-            const bool any_progress = __any(progress);   // as in k_sweep_sf: the limit bounds a stall
-            if (any_progress && spin_limit > 0) spins = 0;
-            if (++spins > spin_limit) {
-                atomicOr(err, 1);
-                return;
-            }
-            if (any_progress) {
-                idle = 0;
-            } else {
-                if (idle < 2) ++idle;
-                if (idle == 1) __builtin_amdgcn_s_sleep(1);
-                else __builtin_amdgcn_s_sleep(2);
-            }
-        }
-    }
-}
 
 template <typename T>
 __global__ void k_sf_prepare(int n, T* __restrict__ a, T* __restrict__ b, int* __restrict__ tickets) {
@@ -615,6 +509,113 @@ __global__ void k_gather_pad<zc, fc>(int64_t n, const int* __restrict__ map, con
     }
 }
 
+// Several right-hand sides with the columns of a row side by side (entry
+// (i, c) at i * kb + c, kb a power of two <= 64).  A wave covers 64 / kb
+// consecutive level-order positions, lane = (row slot, column): the lanes
+// of one row load the same entries (one broadcast access), poll their
+// columns of an input row together (kb adjacent values) and store a row's
+// results side by side.  A workgroup of blockDim / 64 waves takes one
+// ticket for that many consecutive groups of positions (tickets in level
+// order, so the sweep is deadlock-free as k_sweep_sf is; fewer tickets than
+// one per wave, whose atomics on one counter would otherwise bound the
+// sweep).  Each column's result is formed exactly as the one-column sweep
+// forms it.  A wave that reaches the spin limit sets the error flag; its
+// workgroup leaves together at the next ticket.
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(sizeof(T) == 8 ? 512 : 256) void k_sweep_il(int n, int nticket, int kb_log2, int* __restrict__ ticket,
+                                                   int* __restrict__ err, long spin_limit,
+                                                   const int* __restrict__ srow, const int* __restrict__ slice_off,
+                                                   const int* __restrict__ rlen, const int* __restrict__ ecol,
+                                                   const F* __restrict__ eval, const F* __restrict__ dval,
+                                                   const T* __restrict__ rhs, const T* __restrict__ aux,
+                                                   const T* __restrict__ dd, T* __restrict__ x) {
+    __shared__ int s_ticket, s_abort;
+    const int lane = threadIdx.x & (kSfRows - 1), wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    const int kb = 1 << kb_log2;
+    const int col = lane & (kb - 1);
+    const int rows = kSfRows >> kb_log2;   // positions per wave
+    const int slot = lane >> kb_log2;
+    if (threadIdx.x == 0) s_abort = 0;
+    long spins = 0;
+    while (true) {
+        __syncthreads();   // every wave is done with the previous ticket
+        if (threadIdx.x == 0) s_ticket = s_abort ? nticket : atomicAdd(ticket, 1);
+        __syncthreads();
+        const int t = s_ticket;
+        if (t >= nticket) return;
+        const int p = (t * waves + wave) * rows + slot;
+        const bool active = p < n;
+        const int i = active ? srow[p] : 0;
+        const int len = active ? rlen[p] : 0;
+        const int base = active ? slice_off[p / kSfRows] + (p % kSfRows) : 0;
+        unsigned cols[KMAX];
+        F vals[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {   // branch-free, as in RowState::load
+            const int at = k < len ? base + kSfRows * k : 0;
+            cols[k] = (unsigned)ecol[at];
+            vals[k] = eval[at];
+        }
+        T rv = Num<T>::zero(), d_i = Num<T>::zero();
+        F dg = F();
+        if (active) {
+            if (MODE == BWD_IC) {
+                d_i = dd[i];
+                rv = aux[((size_t)i << kb_log2) + col];
+            } else {
+                dg = dval[p];
+                rv = rhs[((size_t)i << kb_log2) + col];
+            }
+        }
+        T g[KMAX];
+        unsigned miss = active ? (len >= 32 ? ~0u : ((1u << len) - 1u)) : 0u;
+        bool done = !active;
+        int idle = 0;
+        while (true) {
+            __asm__ __volatile__("" ::: "memory");
+            bool progress = false;
+            if (!done && miss) {
+                const unsigned want = miss;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (want & (1u << k)) g[k] = gload(x + ((size_t)cols[k] << kb_log2) + col);
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if ((want & (1u << k)) && !sent(g[k])) miss &= ~(1u << k);
+                progress = miss != want;
+            }
+            if (!done && miss == 0u) {
+                T sum = (MODE == BWD_IC) ? Num<T>::zero() : rv;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (k < len) sum = Num<T>::sub(sum, Num<T>::mul(wid<T, F>(vals[k]), g[k]));
+                const T out = (MODE == BWD_IC) ? Num<T>::add(Num<T>::mul(sum, d_i), rv)
+                                               : Num<T>::div(sum, wid<T, F>(dg));
+                gstore(x + ((size_t)i << kb_log2) + col, out);
+                done = true;
+                progress = true;
+            }
+            if (__all(done)) break;
+            const bool any_progress = __any(progress);   // as in k_sweep_sf: the limit bounds a stall
+            if (any_progress && spin_limit > 0) spins = 0;
+            if (++spins > spin_limit) {
+                if (lane == 0) {
+                    atomicOr(err, 1);
+                    s_abort = 1;
+                }
+                break;
+            }
+            if (any_progress) {
+                idle = 0;
+            } else {
+                if (idle < 2) ++idle;
+                if (idle == 1) __builtin_amdgcn_s_sleep(1);
+                else __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+}
+
 inline int grid_for(int64_t n) {
     int64_t g = (n + kBS - 1) / kBS;
     if (g < 1) g = 1;
@@ -658,21 +659,6 @@ void batch_sweep_kmax(hipStream_t s, int kmax, int g, int norder, const int2* or
 #undef SSGPU_BS
 }
 
-template <typename T, typename F, int MODE, int KC>
-void sweep_block_kmax(hipStream_t s, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
-                      long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
-                      const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
-#define SSGPU_SB(K)                                                                                                \
-    hipLaunchKernelGGL((k_sweep_block<T, F, MODE, K, KC>), g, dim3(kSfRows), 0, s, n, nchunks, ncol, tickets, err, \
-                       spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
-    if (kmax <= 8) SSGPU_SB(8);
-    else if (kmax <= 12) SSGPU_SB(12);
-    else if (kmax <= 16) SSGPU_SB(16);
-    else if (kmax <= 20) SSGPU_SB(20);
-    else if (kmax <= 24) SSGPU_SB(24);
-    else SSGPU_SB(32);
-#undef SSGPU_SB
-}
 
 }  // namespace
 
@@ -731,36 +717,46 @@ void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int*
     }
 }
 
-template <typename T, typename F, int KC>
-void sweep_block_kc(hipStream_t s, int mode, int kmax, dim3 g, int n, int nchunks, int ncol, int* tickets, int* err,
-                    long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
-                    const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
-    if (mode == FWD_DIV_LAST)
-        sweep_block_kmax<T, F, FWD_DIV_LAST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
-                                                 slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
-    else if (mode == BWD_IC)
-        sweep_block_kmax<T, F, BWD_IC, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
-                                           slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
-    else
-        sweep_block_kmax<T, F, BWD_DIV_FIRST, KC>(s, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow,
-                                                  slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
+
+
+template <typename T, typename F, int MODE>
+void sweep_il_kmax(hipStream_t s, int kmax, int g, int wpb, int n, int nticket, int kb_log2, int* ticket, int* err,
+                   long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                   const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+#define SSGPU_IL(K)                                                                                               \
+    hipLaunchKernelGGL((k_sweep_il<T, F, MODE, K>), dim3(g), dim3(kSfRows * wpb), 0, s, n, nticket, kb_log2, ticket, \
+                       err, spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x)
+    if (kmax <= 8) SSGPU_IL(8);
+    else if (kmax <= 12) SSGPU_IL(12);
+    else if (kmax <= 16) SSGPU_IL(16);
+    else if (kmax <= 20) SSGPU_IL(20);
+    else if (kmax <= 24) SSGPU_IL(24);
+    else SSGPU_IL(32);
+#undef SSGPU_IL
 }
 
 template <typename T, typename F>
-void launch_sweep_block(hipStream_t s, int mode, int kmax, int n, int grid, int ncol, int kc, int* tickets, int* err,
-                        long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
-                        const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
-    const int nchunks = (n + kSfRows - 1) / kSfRows;
-    const dim3 g((unsigned)std::max(1, std::min(grid, nchunks)), (unsigned)(ncol / kc));
-    if (kc == 4 && sizeof(T) == 8)
-        sweep_block_kc<T, F, 4>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
-                                rlen, ecol, eval, dval, rhs, aux, dd, x);
-    else if (kc == 2)
-        sweep_block_kc<T, F, 2>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
-                                rlen, ecol, eval, dval, rhs, aux, dd, x);
+void launch_sweep_il(hipStream_t s, int mode, int kmax, int n, int grid, int kb, int* ticket, int* err,
+                     long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecol,
+                     const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* x) {
+    int kb_log2 = 0;
+    while ((1 << kb_log2) < kb) ++kb_log2;
+    const int rows = kSfRows >> kb_log2;          // positions per wave
+    // waves per workgroup: up to 32 positions per ticket, at most 8 waves
+    // (4 complex: their registers)
+    const int wpb = std::max(1, std::min(sizeof(T) == 8 ? 8 : 4, 32 / rows));
+    const int ngroups = (n + rows - 1) / rows;
+    const int nticket = (ngroups + wpb - 1) / wpb;
+    const int g = std::max(1, std::min(grid / wpb, nticket));   // grid: waves in all
+    if (mode == FWD_DIV_LAST)
+        sweep_il_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, wpb, n, nticket, kb_log2, ticket, err, spin_limit, srow,
+                                          slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
+    else if (mode == BWD_IC)
+        sweep_il_kmax<T, F, BWD_IC>(s, kmax, g, wpb, n, nticket, kb_log2, ticket, err, spin_limit, srow, slice_off,
+                                    rlen, ecol, eval, dval, rhs, aux, dd, x);
     else
-        sweep_block_kc<T, F, 1>(s, mode, kmax, g, n, nchunks, ncol, tickets, err, spin_limit, srow, slice_off,
-                                rlen, ecol, eval, dval, rhs, aux, dd, x);
+        sweep_il_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, wpb, n, nticket, kb_log2, ticket, err, spin_limit, srow,
+                                           slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
 template <typename T, typename F>
@@ -845,18 +841,6 @@ template void launch_sweep_sf<zc, zc>(hipStream_t, int, int, int, int, int*, int
 template void launch_sweep_sf<zc, fc>(hipStream_t, int, int, int, int, int*, int*, long, const int*, const int*,
                                       const int*, const int*, const fc*, const fc*, const zc*, const zc*, const zc*,
                                       int, zc*);
-template void launch_sweep_block<double, double>(hipStream_t, int, int, int, int, int, int, int*, int*, long,
-                                                 const int*, const int*, const int*, const int*, const double*,
-                                                 const double*, const double*, const double*, const double*, double*);
-template void launch_sweep_block<double, float>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
-                                                const int*, const int*, const int*, const float*, const float*,
-                                                const double*, const double*, const double*, double*);
-template void launch_sweep_block<zc, zc>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
-                                         const int*, const int*, const int*, const zc*, const zc*, const zc*, const zc*,
-                                         const zc*, zc*);
-template void launch_sweep_block<zc, fc>(hipStream_t, int, int, int, int, int, int, int*, int*, long, const int*,
-                                         const int*, const int*, const int*, const fc*, const fc*, const zc*, const zc*,
-                                         const zc*, zc*);
 template void launch_sweep_levels<double, double>(hipStream_t, int, int, int, const int*, const int*, const int*,
                                              const int*, const int*, const double*, const double*, const double*, const double*,
                                              const double*, double*);
@@ -877,6 +861,14 @@ SSGPU_TLI(double, double)
 SSGPU_TLI(double, float)
 SSGPU_TLI(zc, zc)
 SSGPU_TLI(zc, fc)
+#define SSGPU_ILI(T, F)                                                                                     \
+    template void launch_sweep_il<T, F>(hipStream_t, int, int, int, int, int, int*, int*, long, const int*,      \
+                                        const int*, const int*, const int*, const F*, const F*, const T*, const T*, \
+                                        const T*, T*);
+SSGPU_ILI(double, double)
+SSGPU_ILI(double, float)
+SSGPU_ILI(zc, zc)
+SSGPU_ILI(zc, fc)
 #define SSGPU_BSI(T, F)                                                                                   \
     template void launch_batch_sweep<T, F>(hipStream_t, int, int, int, int, const int2*, int*, int*, long,       \
                                            const BatchSys*, const BatchState*, int, int, int, int, GraphRec*);
