@@ -1333,7 +1333,8 @@ int solve_factored(Context* c, Matrix& m, int solver, const ssgpu_options& o, do
 }
 
 // Block solves: nrhs right-hand sides against the cached preparation, in
-// kb >= nrhs padded columns (each stored whole: column c at offset c * n).
+// kb >= nrhs padded columns, a row's columns side by side (entry (i, c) at
+// i * kb + c; the caller's vectors are transposed in and out).
 // Column j runs exactly the solve iccg_main (run), mrtr_main or sgs_main
 // (run_mrtr) runs on it alone (same kernels per column, same reduction
 // trees, same host recurrences and bookkeeping); a column that converges or
