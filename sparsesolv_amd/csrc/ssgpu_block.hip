@@ -48,121 +48,90 @@ template <int NA> __device__ inline void block_sum_multi(double* acc, double* ou
     }
 }
 
-__device__ inline double shfl_xor1(double v) { return __shfl_xor(v, 1); }
-__device__ inline zc shfl_xor1(zc v) { return zc{__shfl_xor(v.re, 1), __shfl_xor(v.im, 1)}; }
 __device__ inline double ntl(const double* p) { return __builtin_nontemporal_load(p); }
 __device__ inline zc ntl(const zc* p) {
     const double* d = reinterpret_cast<const double*>(p);
     return zc{__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1)};
 }
 
-// y = A x for the kb columns; the block's entries are staged in LDS once and
-// every column is summed over them as the one-column kernel sums it (two
-// threads per row: the even- and odd-position chains, y = ta + tb).  Q
-// adjacent columns at a time (64 bytes of an input row): their gathers are
-// issued together, each column's sum still in its own order.  With dotv, one
-// partial per SpMV block and column of dotv_c . y_c, row j's term in thread
-// slot j as in the one-column kernel (the Q columns' partials reduced in one
-// pass, each over the same tree).
-template <typename T, int Q>
-__global__ __launch_bounds__(kBS) void k_spmv_il(int kb, const int* __restrict__ blk, const int* __restrict__ rp,
+// y = A x for the kb columns, one lane per (row, column): the kb lanes of a
+// row read an input row's columns in one access (kb adjacent values).  The
+// block's entries are staged in LDS once; a lane sums its row's even- and
+// odd-position chains (the one-column kernel's two threads per row) in
+// order and stores y = ta + tb, so every column is summed exactly as the
+// one-column kernel sums it.
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_spmv_rc(int kb_log2, const int* __restrict__ blk, const int* __restrict__ rp,
                                                  const int* __restrict__ ci, const T* __restrict__ v,
-                                                 const T* __restrict__ x, T* __restrict__ y,
-                                                 const T* __restrict__ dotv, double* __restrict__ partial) {
-    constexpr int P = BParts<T>::K;
+                                                 const T* __restrict__ x, T* __restrict__ y) {
     __shared__ int s_ci[kSpmvCap];
     __shared__ T s_v[kSpmvCap];
-    __shared__ T s_term[Q][kSpmvRows];
     const int b = blockIdx.x;
     const int r0 = blk[b], r1 = blk[b + 1];
     const int e0 = rp[r0], e1 = rp[r1];
     const int ne = e1 - e0;
-    const int rows = r1 - r0;
-    const int j = (int)threadIdx.x >> 1, par = (int)threadIdx.x & 1;
-    const int r = r0 + j;
-    const bool staged = ne <= kSpmvCap;
-    if (staged) {
+    const int kb = 1 << kb_log2;
+    const int c = (int)threadIdx.x & (kb - 1);
+    const int rows_per_pass = kBS >> kb_log2;
+    if (ne <= kSpmvCap) {
         for (int k = threadIdx.x; k < ne; k += kBS) {
             s_ci[k] = __builtin_nontemporal_load(ci + e0 + k);
             s_v[k] = ntl(v + e0 + k);
         }
         __syncthreads();
-    }
-    for (int c0 = 0; c0 < kb; c0 += Q) {   // kb is a multiple of Q
-        T yv[Q];
+        for (int r = r0 + ((int)threadIdx.x >> kb_log2); r < r1; r += rows_per_pass) {
+            const int e = rp[r + 1] - e0;
+            int k = rp[r] - e0;
+            T ta = Num<T>::zero(), tb = Num<T>::zero();
+            // both chains in order, a few entries' gathers issued together
+            for (; k + 7 < e; k += 8) {
+                T g[8];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) yv[q] = Num<T>::zero();
-        if (staged) {
-            T acc[Q];
+                for (int u = 0; u < 8; ++u) g[u] = x[((size_t)s_ci[k + u] << kb_log2) + c];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) acc[q] = Num<T>::zero();
-            if (r < r1) {
-                const int e = rp[r + 1] - e0;
-                int k = rp[r] - e0 + par;
-                // gathers of a batch of UB entries issued together, then
-                // summed in order
-                constexpr int UB = Q >= 8 ? 4 : 8;
-                for (; k + 2 * (UB - 1) < e; k += 2 * UB) {
-                    T g[UB][Q];
-#pragma unroll
-                    for (int u = 0; u < UB; ++u) {
-                        const T* xr = x + (size_t)s_ci[k + 2 * u] * kb + c0;
-#pragma unroll
-                        for (int q = 0; q < Q; ++q) g[u][q] = xr[q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < Q; ++q)
-#pragma unroll
-                        for (int u = 0; u < UB; ++u) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k + 2 * u], g[u][q]));
-                }
-                for (; k < e; k += 2) {
-                    const T* xr = x + (size_t)s_ci[k] * kb + c0;
-                    T g[Q];
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) g[q] = xr[q];
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) acc[q] = Num<T>::add(acc[q], Num<T>::mul(s_v[k], g[q]));
+                for (int u = 0; u < 8; u += 2) {
+                    ta = Num<T>::add(ta, Num<T>::mul(s_v[k + u], g[u]));
+                    tb = Num<T>::add(tb, Num<T>::mul(s_v[k + u + 1], g[u + 1]));
                 }
             }
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const T other = shfl_xor1(acc[q]);
-                if (r < r1 && par == 0) {
-                    yv[q] = Num<T>::add(acc[q], other);
-                    y[(size_t)r * kb + c0 + q] = yv[q];
-                }
+            for (; k < e; ++k) {
+                const T prod = Num<T>::mul(s_v[k], x[((size_t)s_ci[k] << kb_log2) + c]);
+                if (((k - (rp[r] - e0)) & 1) == 0) ta = Num<T>::add(ta, prod);
+                else tb = Num<T>::add(tb, prod);
             }
-        } else if (threadIdx.x == 0 && r0 < r1) {   // one long row per block
-            for (int q = 0; q < Q; ++q) {
-                const int e = rp[r0 + 1];
-                int k = rp[r0];
-                T ta = Num<T>::zero(), tb = Num<T>::zero();
-                while (k < e) {
-                    ta = Num<T>::add(ta, Num<T>::mul(v[k], x[(size_t)ci[k] * kb + c0 + q]));
-                    ++k;
-                    if (k < e) {
-                        tb = Num<T>::add(tb, Num<T>::mul(v[k], x[(size_t)ci[k] * kb + c0 + q]));
-                        ++k;
-                    }
-                }
-                yv[q] = Num<T>::add(ta, tb);
-                y[(size_t)r0 * kb + c0 + q] = yv[q];
+            y[((size_t)r << kb_log2) + c] = Num<T>::add(ta, tb);
+        }
+    } else if ((int)threadIdx.x < kb && r0 < r1) {   // one long row per block
+        const int e = rp[r0 + 1];
+        int k = rp[r0];
+        T ta = Num<T>::zero(), tb = Num<T>::zero();
+        while (k < e) {
+            ta = Num<T>::add(ta, Num<T>::mul(v[k], x[((size_t)ci[k] << kb_log2) + c]));
+            ++k;
+            if (k < e) {
+                tb = Num<T>::add(tb, Num<T>::mul(v[k], x[((size_t)ci[k] << kb_log2) + c]));
+                ++k;
             }
         }
-        if (dotv) {
-            if (r < r1 && par == 0)
-#pragma unroll
-                for (int q = 0; q < Q; ++q) s_term[q][j] = Num<T>::mul(dotv[(size_t)r * kb + c0 + q], yv[q]);
-            __syncthreads();
-            double acc[Q * P] = {};
-            if ((int)threadIdx.x < rows)
-#pragma unroll
-                for (int q = 0; q < Q; ++q) BParts<T>::put(acc + q * P, s_term[q][threadIdx.x]);
-            // partials of columns c0 .. c0+Q-1 of this block are adjacent
-            block_sum_multi<Q * P>(acc, partial + ((size_t)b * kb + c0) * P);
-            __syncthreads();
-        }
+        y[((size_t)r0 << kb_log2) + c] = Num<T>::add(ta, tb);
     }
+}
+
+// the fused p.Ap of the one-column SpMV: per SpMV block and column, the
+// block tree (row j's term in thread slot j) of dotv_c(r) * y_c(r)
+template <typename T, int KB>
+__global__ __launch_bounds__(kBS) void k_spmv_dot_rc(const int* __restrict__ blk, const T* __restrict__ dotv,
+                                                     const T* __restrict__ y, double* __restrict__ partial) {
+    constexpr int P = BParts<T>::K;
+    const int b = blockIdx.x;
+    const int r0 = blk[b], rows = blk[b + 1] - r0;
+    double acc[KB * P] = {};
+    if ((int)threadIdx.x < rows) {
+        const size_t o = (size_t)(r0 + (int)threadIdx.x) * KB;
+#pragma unroll
+        for (int c = 0; c < KB; ++c) BParts<T>::put(acc + c * P, Num<T>::mul(dotv[o + c], y[o + c]));
+    }
+    block_sum_multi<KB * P>(acc, partial + (size_t)b * KB * P);
 }
 
 // one partial per block and column of a_c . b_c (or |a_c|^2), reduced
@@ -270,9 +239,6 @@ inline int log2_of(int kb) {
     while ((1 << l) < kb) ++l;
     return l;
 }
-// adjacent columns per SpMV pass: 64 bytes of an input row (8 real or 4
-// complex values; fewer when kb is smaller)
-template <typename T> inline int spmv_q(int kb) { return std::min(kb, sizeof(T) == 8 ? 8 : 4); }
 // adjacent columns per thread of the reductions: 128 bytes of a row (16 real
 // or 8 complex values)
 template <typename T> inline int cq_of(int kb) { return std::min(kb, sizeof(T) == 8 ? 16 : 8); }
@@ -282,15 +248,16 @@ template <typename T> inline int cq_of(int kb) { return std::min(kb, sizeof(T) =
 template <typename T>
 void launch_spmv_il(hipStream_t s, int kb, int nblk, const int* blk, const int* rp, const int* ci, const T* v,
                     const T* x, T* y, const T* dotv, double* partial) {
-    const int q = spmv_q<T>(kb);
-    if (q == 8)
-        hipLaunchKernelGGL((k_spmv_il<T, 8>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
-    else if (q == 4)
-        hipLaunchKernelGGL((k_spmv_il<T, 4>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
-    else if (q == 2)
-        hipLaunchKernelGGL((k_spmv_il<T, 2>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
-    else
-        hipLaunchKernelGGL((k_spmv_il<T, 1>), dim3(nblk), dim3(kBS), 0, s, kb, blk, rp, ci, v, x, y, dotv, partial);
+    hipLaunchKernelGGL(k_spmv_rc<T>, dim3(nblk), dim3(kBS), 0, s, log2_of(kb), blk, rp, ci, v, x, y);
+    if (!dotv) return;
+#define SSGPU_SD(K) hipLaunchKernelGGL((k_spmv_dot_rc<T, K>), dim3(nblk), dim3(kBS), 0, s, blk, dotv, y, partial)
+    if (kb == 32) SSGPU_SD(32);
+    else if (kb == 16) SSGPU_SD(16);
+    else if (kb == 8) SSGPU_SD(8);
+    else if (kb == 4) SSGPU_SD(4);
+    else if (kb == 2) SSGPU_SD(2);
+    else SSGPU_SD(1);
+#undef SSGPU_SD
 }
 
 // launch KERNEL(CQ) over (GX, kb / CQ) workgroups
