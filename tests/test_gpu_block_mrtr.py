@@ -85,9 +85,11 @@ def _block_vs_solo(torch, ctx, m, solver, B, X0, opts, max_ite, label):
 
 
 @pytest.mark.parametrize("solver", [capi.MRTR_IC, capi.MRTR_SGS], ids=["ic_mrtr", "sgs_mrtr"])
-@pytest.mark.parametrize("cplx,k", [(False, 1), (False, 3), (False, 8), (True, 2), (True, 5)])
+@pytest.mark.parametrize("cplx,k", [(False, 1), (False, 3), (False, 8), (False, 32), (True, 2), (True, 5), (True, 32)])
 def test_block_mrtr_columns_match_single_solves(torch_cuda, ctx, solver, cplx, k):
-    s = voxel.eddy_current(6, seed=6) if cplx else voxel.curlcurl(9, seed=6)
+    # 32 columns: a wave of the block sweep covers 2 rows (4-wave workgroups
+    # for complex values, 8 for real)
+    s = voxel.eddy_current(6 if k < 32 else 5, seed=6) if cplx else voxel.curlcurl(9 if k < 32 else 7, seed=6)
     m = _prepare(ctx, s, solver, 1.1)
     try:
         B = _columns(s, k)
