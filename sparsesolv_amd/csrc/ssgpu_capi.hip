@@ -231,6 +231,15 @@ static void build_sf(Context* c, int n, const std::vector<int>& order, const SfR
     upload(c, out.slice_off, soff);
     upload(c, out.rlen, rlen);
     upload(c, out.ecol, ecol);
+    if (!out.by_level) {   // the inputs' level-order positions, for the position-published sweep
+        hvec<int> pos(std::max(n, 1));
+        parallel_for(n, [&](int p) { pos[order[p]] = p; });
+        parallel_for(nsl, [&](int sl) {
+            for (int64_t e = soff[sl]; e < soff[sl + 1]; ++e) ecol[e] = pos[ecol[e]];
+        });
+        if (nell == 0) ecol[0] = 0;
+        upload(c, out.ecolp, ecol);
+    }
     if (with_f) {
         upload(c, out.emap_f, ef);
         upload(c, out.dmap_f, df);
@@ -479,6 +488,7 @@ struct Workspace {
     double* partial = nullptr;
     double* scal = nullptr;        // kScalSlots doubles
     int* tickets = nullptr;        // 16 ints
+    char* xpos = nullptr;          // 2 vectors of vec_bytes: the position-published sweeps' polled arrays
 };
 
 struct Solver {
@@ -564,11 +574,21 @@ struct Solver {
         if (c->d_tickets.bytes == 0) c->d_tickets.alloc(c, 64);
         return c->d_tickets.as<int>();
     }
+    // the array a sync-free sweep over S polls when it publishes at level
+    // positions (slot k of two), or null when S sweeps another way
+    template <typename T> T* pos_buf(const SfCsr& S, int k) {
+        if (S.by_level || (S.use_tile && S.tile) || !S.ecolp.bytes) return nullptr;
+        if (ws) return reinterpret_cast<T*>(ws->xpos + ws->vec_bytes * (size_t)k);
+        DevBuf& b = c->d_xpos[k];
+        const size_t need = sizeof(T) * (size_t)std::max(m.n, 1);
+        if (b.bytes < need) b.alloc(c, need);
+        return b.as<T>();
+    }
     int* errflag() { return reinterpret_cast<int*>(scal()) + kErrInt; }
     bool sf_ok() const { return m.sfL.kmax > 0 && m.sfUic.kmax > 0; }
     template <typename T, typename F>
     void sf_sweep(SfCsr& S, int mode, bool reverse, int* ticket, const F* ev, const F* dv, const T* rhs,
-                  const T* aux, const T* dd, T* x) {
+                  const T* aux, const T* dd, T* x, T* xp = nullptr) {
         (void)reverse;   // backward structures are already in their own level order
         if (S.use_tile && S.tile && !S.by_level) {
             const int pair = (sizeof(T) == 16 ? 2 : 0) + (sizeof(F) < sizeof(T) ? 1 : 0);
@@ -581,6 +601,10 @@ struct Solver {
             launch_sweep_levels<T, F>(s, mode, S.kmax, S.lv->nlev, S.lv->levptr.data(), S.srow,
                                       S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux,
                                       dd, x);
+        else if (xp)   // xp: pos_buf of S, sentinel-filled by the caller
+            launch_sweep_sf_pos<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), spin_limit, S.srow,
+                                      S.slice_off.as<int>(), S.rlen.as<int>(), S.ecolp.as<int>(), ev, dv, rhs, aux,
+                                      dd, m.sweep_idle, xp, x);
         else
             launch_sweep_sf<T, F>(s, mode, S.kmax, m.n, m.sweep_grid, ticket, errflag(), spin_limit, S.srow,
                                   S.slice_off.as<int>(), S.rlen.as<int>(), S.ecol.as<int>(), ev, dv, rhs, aux, dd,
@@ -609,17 +633,22 @@ struct Solver {
         using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         if (sf_ok()) {
-            if (!m.sfL.by_level || !m.sfUic.by_level) launch_sf_prepare<T>(s, m.n, tmp, z, tickets());
+            // a sync-free sweep polls its level-position array (xf, xb) and
+            // stores by row; the others poll (tiles) or read their outputs
+            T* xf = pos_buf<T>(m.sfL, 0);
+            T* xb = pos_buf<T>(m.sfUic, 1);
+            if (!m.sfL.by_level || !m.sfUic.by_level)
+                launch_sf_prepare<T>(s, m.n, xf ? xf : tmp, xb ? xb : z, tickets());
             if (f32) {
                 sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r,
-                               nullptr, nullptr, tmp);
+                               nullptr, nullptr, tmp, xf);
                 sf_sweep<T, F>(m.sfUic, BWD_IC, true, tickets() + 1, m.sfUic.eval32.as<F>(), nullptr, nullptr, tmp,
-                               m.diagD.as<T>(), z);
+                               m.diagD.as<T>(), z, xb);
             } else {
                 sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r,
-                               nullptr, nullptr, tmp);
+                               nullptr, nullptr, tmp, xf);
                 sf_sweep<T, T>(m.sfUic, BWD_IC, true, tickets() + 1, m.sfUic.eval.as<T>(), nullptr, nullptr, tmp,
-                               m.diagD.as<T>(), z);
+                               m.diagD.as<T>(), z, xb);
             }
             return;
         }
@@ -637,13 +666,14 @@ struct Solver {
         const bool f32 = o.factor_precision == SSGPU_FACTOR_F32;
         (void)aux;
         if (sf_ok()) {
-            if (!(forward ? m.sfL : m.sfUic).by_level) launch_sf_prepare<T>(s, m.n, out, nullptr, tickets());
+            T* xp = pos_buf<T>(forward ? m.sfL : m.sfUic, 0);
+            if (!(forward ? m.sfL : m.sfUic).by_level) launch_sf_prepare<T>(s, m.n, xp ? xp : out, nullptr, tickets());
             if (forward) {
-                if (f32) sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), in, nullptr, nullptr, out);
-                else sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), in, nullptr, nullptr, out);
+                if (f32) sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), in, nullptr, nullptr, out, xp);
+                else sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), in, nullptr, nullptr, out, xp);
             } else {
-                if (f32) sf_sweep<T, F>(m.sfUic, BWD_IC, true, tickets(), m.sfUic.eval32.as<F>(), nullptr, nullptr, in, m.diagD.as<T>(), out);
-                else sf_sweep<T, T>(m.sfUic, BWD_IC, true, tickets(), m.sfUic.eval.as<T>(), nullptr, nullptr, in, m.diagD.as<T>(), out);
+                if (f32) sf_sweep<T, F>(m.sfUic, BWD_IC, true, tickets(), m.sfUic.eval32.as<F>(), nullptr, nullptr, in, m.diagD.as<T>(), out, xp);
+                else sf_sweep<T, T>(m.sfUic, BWD_IC, true, tickets(), m.sfUic.eval.as<T>(), nullptr, nullptr, in, m.diagD.as<T>(), out, xp);
             }
             return;
         }
@@ -661,13 +691,14 @@ struct Solver {
     template <typename T> void fr(const T* r, T* z) {
         using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         if (m.sfL.kmax > 0) {
-            if (!m.sfL.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
+            T* xp = pos_buf<T>(m.sfL, 0);
+            if (!m.sfL.by_level) launch_sf_prepare<T>(s, m.n, xp ? xp : z, nullptr, tickets());
             if (o.factor_precision == SSGPU_FACTOR_F32)
                 sf_sweep<T, F>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval32.as<F>(), m.sfL.dval32.as<F>(), r,
-                               nullptr, nullptr, z);
+                               nullptr, nullptr, z, xp);
             else
                 sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(), r,
-                               nullptr, nullptr, z);
+                               nullptr, nullptr, z, xp);
         } else {
             sweep<T, T>(m.L, FWD_DIV_LAST, m.Lv.as<T>(), r, nullptr, nullptr, z);
         }
@@ -675,13 +706,14 @@ struct Solver {
     template <typename T> void bc(const T* r, T* z) {
         using F = typename std::conditional<std::is_same<T, double>::value, float, fc>::type;
         if (m.sfUs.kmax > 0) {
-            if (!m.sfUs.by_level) launch_sf_prepare<T>(s, m.n, z, nullptr, tickets());
+            T* xp = pos_buf<T>(m.sfUs, 0);
+            if (!m.sfUs.by_level) launch_sf_prepare<T>(s, m.n, xp ? xp : z, nullptr, tickets());
             if (o.factor_precision == SSGPU_FACTOR_F32)
                 sf_sweep<T, F>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval32.as<F>(), m.sfUs.dval32.as<F>(),
-                               r, nullptr, nullptr, z);
+                               r, nullptr, nullptr, z, xp);
             else
                 sf_sweep<T, T>(m.sfUs, BWD_DIV_FIRST, true, tickets(), m.sfUs.eval.as<T>(), m.sfUs.dval.as<T>(), r,
-                               nullptr, nullptr, z);
+                               nullptr, nullptr, z, xp);
         } else {
             sweep<T, T>(m.Us, BWD_DIV_FIRST, m.Uv.as<T>(), r, nullptr, nullptr, z);
         }
@@ -780,7 +812,7 @@ struct Solver {
             if (cand.size() == 1 && want_tile) apply(cand[0]);
             return;
         }
-        DevBuf r, out;
+        DevBuf r, out, xpl;   // freed on return: factorization leaves the context's pool as it was
         r.alloc(c, sizeof(T) * m.n);
         out.alloc(c, sizeof(T) * m.n);
         launch_fill<T>(s, m.n, r.as<T>(), one<T>());
@@ -793,10 +825,15 @@ struct Solver {
             apply(sh);
             float t = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
-                if (sh.kind != 1) launch_sf_prepare<T>(s, m.n, out.as<T>(), nullptr, tickets());
+                T* xp = nullptr;   // the array a position-published sweep polls
+                if (sh.kind == 0 && m.sfL.ecolp.bytes) {
+                    if (xpl.bytes < out.bytes) xpl.alloc(c, out.bytes);
+                    xp = xpl.as<T>();
+                }
+                if (sh.kind != 1) launch_sf_prepare<T>(s, m.n, xp ? xp : out.as<T>(), nullptr, tickets());
                 SSGPU_HIP(hipEventRecord(e0, s));
                 sf_sweep<T, T>(m.sfL, FWD_DIV_LAST, false, tickets(), m.sfL.eval.as<T>(), m.sfL.dval.as<T>(),
-                               r.as<T>(), nullptr, nullptr, out.as<T>());
+                               r.as<T>(), nullptr, nullptr, out.as<T>(), xp);
                 SSGPU_HIP(hipEventRecord(e1, s));
                 SSGPU_HIP(hipEventSynchronize(e1));
                 float ms = 0.0f;
@@ -1969,8 +2006,9 @@ void* scratch_dev(int device) {
 }
 
 // Lay out a solve's work space in the scratch array: the work vectors the
-// solver, its options and the true-residual pass use, the reduction
-// partials, the scalar block and the tickets.  Throws BAD_ARG when it does
+// solver, its options and the true-residual pass use, the two arrays the
+// position-published sweeps poll, the reduction partials, the scalar block
+// and the tickets.  Throws BAD_ARG when it does
 // not fit.
 void plan_workspace(Workspace& w, char* base, const Matrix& m, int solver, const ssgpu_options& o) {
     std::vector<int> idx;
@@ -1991,7 +2029,8 @@ void plan_workspace(Workspace& w, char* base, const Matrix& m, int solver, const
     const size_t scal_b = up(kScalSlots * sizeof(double)), tick_b = 256;
     const size_t part_b = up(sizeof(double) * 8 * (size_t)std::max(m.nblk, kMaxReduceBlocks));
     w.vec_bytes = up(m.vsize * (size_t)std::max(m.n, 1));
-    const size_t need = scal_b + tick_b + part_b + w.vec_bytes * (size_t)ns;
+    // plus the two arrays the position-published sweeps poll
+    const size_t need = scal_b + tick_b + part_b + w.vec_bytes * (size_t)(ns + 2);
     if (need > kScratchBytes)
         throw Error{BAD_ARG, "the allocation-free solve needs " + std::to_string(need) + " bytes of work space for "
                              "this system (n = " + std::to_string(m.n) + ", " + std::to_string(ns) +
@@ -2000,6 +2039,7 @@ void plan_workspace(Workspace& w, char* base, const Matrix& m, int solver, const
     w.tickets = reinterpret_cast<int*>(base + scal_b);
     w.partial = reinterpret_cast<double*>(base + scal_b + tick_b);
     w.vecs = base + scal_b + tick_b + part_b;
+    w.xpos = w.vecs + w.vec_bytes * (size_t)ns;
 }
 
 }  // namespace ssgpu
@@ -2113,6 +2153,8 @@ int ssgpu_context_trim(ssgpu_context ctx) {
     c.d_bpartial.release();
     c.d_btickets.release();
     c.d_bscal.release();
+    c.d_xpos[0].release();
+    c.d_xpos[1].release();
     return SSGPU_OK;
     SSGPU_GUARD_END
 }
@@ -2650,7 +2692,6 @@ int ssgpu_time_kernel(ssgpu_matrix mh, int which, int reps, double accel, int fa
     o.factor_precision = factor_precision;
     Solver S(m, o);
     if (which >= 1 && which <= 3 && !m.factor_valid) throw Error{BAD_ARG, "no factor"};
-    if (which == 4 && !m.lower_diag_ok) throw Error{NO_DIAGONAL, "there is not diagonal element!"};
     hipEvent_t e0, e1;
     SSGPU_HIP(hipEventCreate(&e0));
     SSGPU_HIP(hipEventCreate(&e1));

@@ -39,6 +39,7 @@ struct Context {
     DevBuf d_scal;              // device scalar results; the last slot holds the sweep error flag
     DevBuf d_count;
     DevBuf d_tickets;           // sync-free sweep tickets (2 ints)
+    DevBuf d_xpos[2];           // the arrays the position-published sweeps poll (n values each)
     std::mutex mu;
     // matrices created on this context and not yet destroyed; a context whose
     // destroy arrives while some are alive is freed with the last of them
@@ -150,6 +151,7 @@ struct SfCsr {
     const LevelCsr* lv = nullptr;   // its levels (host level starts)
     bool by_level = false;       // one launch per level instead of the sync-free sweep
     DevBuf slice_off, rlen, ecol, emap_f, dmap_f, emap_a, dmap_a;
+    DevBuf ecolp;   // ecol as level-order positions (sync-free structures: the position-published sweep)
     DevBuf eval, dval, eval32, dval32;
     std::vector<int> h_soff;     // slice offsets (host copy, for the tile layout's entry indices)
 };

@@ -174,6 +174,13 @@ template <typename T, typename F>
 void launch_sweep_levels(hipStream_t s, int mode, int kmax, int nlev, const int* levptr, const int* srow,
                          const int* slice_off, const int* rlen, const int* ecol, const F* eval, const F* dval,
                          const T* rhs, const T* aux, const T* dd, T* x);
+// the same sweep publishing results at level positions: xp (n values,
+// sentinel-filled) is polled, ecolp gives the inputs' positions, and xo
+// receives the results by row
+template <typename T, typename F>
+void launch_sweep_sf_pos(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, long spin_limit,
+                         const int* srow, const int* slice_off, const int* rlen, const int* ecolp, const F* eval,
+                         const F* dval, const T* rhs, const T* aux, const T* dd, int max_idle, T* xp, T* xo);
 // block sweep over kb right-hand sides with entry (i, c) at i * kb + c (kb a
 // power of two <= 32; rhs, aux and x in that layout); grid: waves in all;
 // ticket: one zeroed counter

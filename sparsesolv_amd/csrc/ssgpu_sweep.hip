@@ -133,7 +133,7 @@ __device__ inline bool sf_chunk(int c, int n, int* __restrict__ err, long spin_l
                                 const int* __restrict__ slice_off, const int* __restrict__ rlen,
                                 const int* __restrict__ ecol, const F* __restrict__ eval, const F* __restrict__ dval,
                                 const T* __restrict__ rhs, const T* __restrict__ aux, const T* __restrict__ dd,
-                                int max_idle, T* __restrict__ x, long& spins) {
+                                int max_idle, T* __restrict__ x, long& spins, T* __restrict__ xo = nullptr) {
     const int lane = threadIdx.x;
     const int p = c * kSfRows + lane;   // position in level order
     const bool active = p < n;
@@ -159,7 +159,13 @@ __device__ inline bool sf_chunk(int c, int n, int* __restrict__ err, long spin_l
             progress = miss != want;
         }
         if (!done && miss == 0ull) {
-            gstore(x + row.i, row.result(g));
+            const T res = row.result(g);
+            if (xo) {   // published at the level position, and stored by row
+                gstore(x + p, res);
+                xo[row.i] = res;
+            } else {
+                gstore(x + row.i, res);
+            }
             done = true;
             progress = true;
         }
@@ -203,6 +209,30 @@ __global__ __launch_bounds__(kSfRows) void k_sweep_sf(int n, int nchunks, int* _
         if (c >= nchunks) return;
         if (!sf_chunk<T, F, MODE, KMAX>(c, n, err, spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd,
                                         max_idle, x, spins))
+            return;
+    }
+}
+
+// The same sweep publishing each result at its level-order position in xp
+// (the array the waves poll, inputs given by their positions in ecol) and
+// storing it by row in xo.  A chunk's rows then read, per input direction,
+// adjacent positions of the previous levels, so a wave's polls touch a few
+// cache lines instead of one per lane; the arithmetic is unchanged.
+template <typename T, typename F, int MODE, int KMAX>
+__global__ __launch_bounds__(kSfRows) void k_sweep_sf_pos(int n, int nchunks, int* __restrict__ ticket,
+                                                          int* __restrict__ err, long spin_limit,
+                                                          const int* __restrict__ srow,
+                                                          const int* __restrict__ slice_off, const int* __restrict__ rlen,
+                                                          const int* __restrict__ ecol, const F* __restrict__ eval,
+                                                          const F* __restrict__ dval, const T* __restrict__ rhs,
+                                                          const T* __restrict__ aux, const T* __restrict__ dd,
+                                                          int max_idle, T* __restrict__ xp, T* __restrict__ xo) {
+    long spins = 0;
+    while (true) {
+        const int c = take_ticket(ticket);
+        if (c >= nchunks) return;
+        if (!sf_chunk<T, F, MODE, KMAX>(c, n, err, spin_limit, srow, slice_off, rlen, ecol, eval, dval, rhs, aux, dd,
+                                        max_idle, xp, spins, xo))
             return;
     }
 }
@@ -759,6 +789,40 @@ void launch_sweep_il(hipStream_t s, int mode, int kmax, int n, int grid, int kb,
                                            slice_off, rlen, ecol, eval, dval, rhs, aux, dd, x);
 }
 
+template <typename T, typename F, int MODE>
+void sweep_pos_kmax(hipStream_t s, int kmax, int g, int max_idle, int n, int nchunks, int* ticket, int* err,
+                    long spin_limit, const int* srow, const int* slice_off, const int* rlen, const int* ecolp,
+                    const F* eval, const F* dval, const T* rhs, const T* aux, const T* dd, T* xp, T* xo) {
+#define SSGPU_SP(K)                                                                                               \
+    hipLaunchKernelGGL((k_sweep_sf_pos<T, F, MODE, K>), dim3(g), dim3(kSfRows), 0, s, n, nchunks, ticket, err,      \
+                       spin_limit, srow, slice_off, rlen, ecolp, eval, dval, rhs, aux, dd, max_idle, xp, xo)
+    if (kmax <= 8) SSGPU_SP(8);
+    else if (kmax <= 12) SSGPU_SP(12);
+    else if (kmax <= 16) SSGPU_SP(16);
+    else if (kmax <= 20) SSGPU_SP(20);
+    else if (kmax <= 24) SSGPU_SP(24);
+    else if (kmax <= 32) SSGPU_SP(32);
+    else SSGPU_SP(64);
+#undef SSGPU_SP
+}
+
+template <typename T, typename F>
+void launch_sweep_sf_pos(hipStream_t s, int mode, int kmax, int n, int grid, int* ticket, int* err, long spin_limit,
+                         const int* srow, const int* slice_off, const int* rlen, const int* ecolp, const F* eval,
+                         const F* dval, const T* rhs, const T* aux, const T* dd, int max_idle, T* xp, T* xo) {
+    const int nchunks = (n + kSfRows - 1) / kSfRows;
+    const int g = std::max(1, std::min(grid, nchunks));
+    if (mode == FWD_DIV_LAST)
+        sweep_pos_kmax<T, F, FWD_DIV_LAST>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow,
+                                           slice_off, rlen, ecolp, eval, dval, rhs, aux, dd, xp, xo);
+    else if (mode == BWD_IC)
+        sweep_pos_kmax<T, F, BWD_IC>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow, slice_off,
+                                     rlen, ecolp, eval, dval, rhs, aux, dd, xp, xo);
+    else
+        sweep_pos_kmax<T, F, BWD_DIV_FIRST>(s, kmax, g, max_idle, n, nchunks, ticket, err, spin_limit, srow,
+                                            slice_off, rlen, ecolp, eval, dval, rhs, aux, dd, xp, xo);
+}
+
 template <typename T, typename F>
 void launch_batch_sweep(hipStream_t s, int mode, int kmax, int grid, int norder, const int2* order, int* ticket,
                         int* err, long spin_limit, const BatchSys* sys, const BatchState* st, int want, int in,
@@ -861,6 +925,14 @@ SSGPU_TLI(double, double)
 SSGPU_TLI(double, float)
 SSGPU_TLI(zc, zc)
 SSGPU_TLI(zc, fc)
+#define SSGPU_SPI(T, F)                                                                                       \
+    template void launch_sweep_sf_pos<T, F>(hipStream_t, int, int, int, int, int*, int*, long, const int*,         \
+                                            const int*, const int*, const int*, const F*, const F*, const T*,     \
+                                            const T*, const T*, int, T*, T*);
+SSGPU_SPI(double, double)
+SSGPU_SPI(double, float)
+SSGPU_SPI(zc, zc)
+SSGPU_SPI(zc, fc)
 #define SSGPU_ILI(T, F)                                                                                     \
     template void launch_sweep_il<T, F>(hipStream_t, int, int, int, int, int, int*, int*, long, const int*,      \
                                         const int*, const int*, const int*, const F*, const F*, const T*, const T*, \

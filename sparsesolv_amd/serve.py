@@ -102,20 +102,23 @@ class Response:
 def _system_bytes(nnz: int, n: int, cplx: bool, precision: int = capi.FACTOR_F64) -> int:
     """Upper bound of the device bytes of a resident system with its
     preconditioner: CSR, the level-ordered lower / transposed copies, two
-    sliced-ELL copies, the factor and its update lists (an FP32 factor keeps
-    the FP64 copy too).  Measured on curl-curl and eddy-current systems of
-    240 to 626,580 unknowns, for every solver: at most 0.96 of this bound."""
+    sliced-ELL copies (with their columns also as level positions, for the
+    position-published sweeps), the factor and its update lists (an FP32
+    factor keeps the FP64 copy too).  Before the position columns were added,
+    systems of 240 to 626,580 unknowns measured at most 0.96 of the bound
+    without their 8 bytes per entry, for every solver."""
     f32 = precision == capi.FACTOR_F32
-    per_entry = (130 if f32 else 114) if cplx else (94 if f32 else 84)
+    per_entry = (138 if f32 else 122) if cplx else (102 if f32 else 92)
     return int(per_entry * nnz + 64 * n + (192 << 10))
 
 
 def _pool_bytes(n: int, cplx: bool) -> int:
     """Upper bound of a context's work pool after solving a system of n
-    unknowns: at most 14 vectors (the MRTR solvers' 12 with the best-iterate
-    copy and refinement, plus the two permuted copies of an ABMC-ordered
-    matrix) and the reduction partials."""
-    return 14 * (16 if cplx else 8) * n + (128 << 10)
+    unknowns: at most 16 vectors (the MRTR solvers' 12 with the best-iterate
+    copy and refinement, the two permuted copies of an ABMC-ordered matrix,
+    the two arrays the position-published sweeps poll) and the reduction
+    partials."""
+    return 16 * (16 if cplx else 8) * n + (128 << 10)
 
 
 def _assembly_bytes(ntrip: int, cplx: bool) -> int:
