@@ -590,6 +590,7 @@ def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
     d_y = torch.zeros_like(d_b)
     torch.cuda.synchronize()
     t_spmv = m.time_kernel(0, a.reps, d_b.data_ptr(), d_y.data_ptr())
+    m.time_kernel(4, 1, accel=a.accel, precision=prec)   # the IC factor the timed sweeps use (SGS-MRTR has none)
     t_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
     t_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=prec)
     f32 = None
@@ -608,6 +609,7 @@ def abmc_run(a, s, ctx, d_b, normB, code, prec, opts):
                                  opts=o32)
             lat32.append(time.perf_counter() - t)
         x32 = d_x.cpu().numpy()
+        m.time_kernel(4, 1, accel=a.accel, precision=capi.FACTOR_F32)   # the FP32 IC factor the sweeps use
         f32_fwd = m.time_kernel(1, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=capi.FACTOR_F32)
         f32_bwd = m.time_kernel(2, a.reps, d_b.data_ptr(), d_y.data_ptr(), precision=capi.FACTOR_F32)
         f32 = {"solves_per_s": 1.0 / statistics.median(lat32), "ms_per_solve": 1e3 * statistics.median(lat32),
